@@ -1,0 +1,126 @@
+// fpho_internal.h -- device-side data layout of the MI355X likelihood engine.
+//
+// HBM layout (all structure-of-arrays, per patch unless noted):
+//   pixels   data, ierr, xpix, ypix, residual : float[npix]  (superpixel order,
+//            exposures concatenated, padding pixels carry ierr = 0)
+//   ImgGauss workspace : one 96-byte record per (exposure, source, psf comp)
+//            written by the convert kernel and read by the pixel kernel with
+//            wave-uniform (scalar) loads; the first 32 bytes are all the
+//            residual pass needs.
+//   ExpDesc  : one per exposure (batch-global), 128 bytes.
+//   TileDesc : one per wave tile of 64*K pixels of one exposure.
+//   partials : float[n_src*7] per tile + double chi2 per tile; summed in a
+//              fixed order by the reduce kernel (deterministic results).
+#pragma once
+
+#include <stdint.h>
+#include "../../include/fpho.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define FPHO_HD __host__ __device__
+#else
+#define FPHO_HD
+#endif
+
+namespace fpho {
+
+constexpr int kWave = 64;
+constexpr int kNParams = FPHO_NPARAMS;
+
+// One image-plane Gaussian (a source Sersic component convolved with one PSF
+// component) plus the Jacobian of its 6 image parameters with respect to the
+// 7 on-sky source parameters.  Same quantities as ImageGaussian in
+// forcepho/src/common_kernel.cc:20-46, re-laid-out hot/cold for scalar loads;
+// hc = 1 - (fxx + fyy)/24 folds the constant part of the pixel-integral
+// correction H.
+struct alignas(16) ImgGauss {
+    // hot: residual pass (32 bytes)
+    float xcen, ycen, fxx, fyy, fxy, amp, hc, pad0;
+    // cold: gradient pass (64 bytes)
+    float dA_dFlux, dA_dQ, dFxx_dQ, dFyy_dQ;
+    float dFxy_dQ, dA_dPA, dFxx_dPA, dFyy_dPA;
+    float dFxy_dPA, dA_dSersic, dA_drh, pad1;
+    float dx_dAlpha, dy_dAlpha, dx_dDelta, dy_dDelta;
+};
+static_assert(sizeof(ImgGauss) == 96, "ImgGauss must be 96 bytes");
+
+// Everything the kernels need to know about one exposure of one patch.
+struct alignas(16) ExpDesc {
+    const float *data, *ierr, *xpix, *ypix;  // patch pixel arrays
+    float *residual;                         // nullptr: not written
+    const float *rad2;                       // [n_radii]
+    const float *D, *CW;                     // at (exposure, source 0): [n_src][4]
+    const float *crpix, *crval;              // at (exposure, source 0): [n_src][2]
+    const fpho_psf_gauss *psf;               // psfgauss + psfgauss_start[e]
+    int64_t gauss_off;                       // first ImgGauss of this exposure
+    int32_t src_off;                         // first fpho_source of this patch
+    int32_t band;                            // band index within the patch
+    int32_t n_src, n_psf, n_radii;
+    float G;
+};
+static_assert(sizeof(ExpDesc) == 128, "ExpDesc must be 128 bytes");
+
+// One wave tile: pixels [pix0, pix0 + npix) of exposure `exp`.
+struct alignas(8) TileDesc {
+    int32_t exp;       // ExpDesc index
+    int32_t npix;      // <= 64*K
+    int32_t pix0;      // offset into the patch pixel arrays
+    int32_t pad;
+    int64_t part_off;  // first float of this tile's gradient partials
+};
+static_assert(sizeof(TileDesc) == 24, "TileDesc must be 24 bytes");
+
+// One reduce output: band `band` of a patch, its tiles [tile0, tile0+ntiles)
+// are contiguous and their partials contiguous from part0 (stride n_src*7).
+struct alignas(8) OutDesc {
+    int64_t tile0;
+    int64_t part0;
+    int64_t chi2_out;   // index into chi2 output
+    int64_t dchi2_out;  // index into dchi2 output ([n_src][7] block)
+    int32_t ntiles;
+    int32_t n_src;
+};
+static_assert(sizeof(OutDesc) == 40, "OutDesc must be 40 bytes");
+
+// One active source of the batch (for proposals and gradient stacking).
+struct alignas(8) SrcDesc {
+    int64_t theta_off;   // first parameter of this source
+    int64_t dchi2_base;  // dchi2 index of (band 0, this source, param 0)
+    int32_t n_bands;     // flux parameters of this source
+    int32_t src_stride;  // n_src * 7 of its patch (band stride in dchi2)
+    int32_t proposal;    // global fpho_source index
+    int32_t pad;
+};
+static_assert(sizeof(SrcDesc) == 32, "SrcDesc must be 32 bytes");
+
+struct alignas(8) PatchOut {
+    int64_t chi2_off;    // first band of this patch in chi2 output
+    int32_t n_bands;
+    int32_t pad;
+};
+
+// ------------------------------------------------------------ 2x2 algebra
+// Row-major [[a, b], [c, d]]; the order of forcepho/src/matrix22.cc.
+struct M22 {
+    float a, b, c, d;
+};
+FPHO_HD inline M22 mul(const M22 &x, const M22 &y) {
+    return {x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d,
+            x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+FPHO_HD inline M22 transpose(const M22 &x) { return {x.a, x.c, x.b, x.d}; }
+FPHO_HD inline M22 add(const M22 &x, const M22 &y) {
+    return {x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d};
+}
+FPHO_HD inline M22 scale(const M22 &x, float s) {
+    return {x.a * s, x.b * s, x.c * s, x.d * s};
+}
+FPHO_HD inline float det(const M22 &x) { return x.a * x.d - x.b * x.c; }
+FPHO_HD inline float trace(const M22 &x) { return x.a + x.d; }
+FPHO_HD inline M22 inverse(const M22 &x) {
+    const float id = 1.0f / det(x);
+    return {x.d * id, -x.b * id, -x.c * id, x.a * id};
+}
+
+}  // namespace fpho

@@ -1,0 +1,46 @@
+// kernels.h -- host-side launch wrappers of kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fpho_internal.h"
+
+namespace fpho {
+
+struct EvalArgs {
+    const TileDesc *tiles;
+    const int32_t *queue;  // tile order (cost-sorted)
+    int64_t n_tiles;
+    int32_t *counter;      // persistent work-queue counter (device)
+    const ExpDesc *exps;
+    const ImgGauss *gauss;
+    float *gpart;
+    double *cpart;
+    int pix_per_lane;
+    int waves_per_group;
+    int persistent;
+    int64_t grid;          // persistent: number of workgroups
+};
+
+struct SplineView {
+    const double *coeffs;  // [n_amp][n_tx-4][n_ty-4]
+    const double *tx;
+    const double *ty;
+    const double *radii;
+    int32_t n_tx, n_ty, n_amp, pad;
+    double nsersic_min;
+};
+
+hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
+                          ImgGauss *out, hipStream_t stream);
+hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
+hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,
+                         const double *cpart, double *chi2, double *dchi2, int fp64,
+                         hipStream_t stream);
+hipError_t launch_theta(const SrcDesc *srcs, int64_t n_srcs, const double *theta,
+                        const SplineView &sp, fpho_source *props, hipStream_t stream);
+hipError_t launch_stack(const SrcDesc *srcs, int64_t n_srcs, const PatchOut *pouts,
+                        int64_t n_patches, const double *chi2, const double *dchi2,
+                        double *lnlike, double *grad, hipStream_t stream);
+
+}  // namespace fpho

@@ -1,0 +1,779 @@
+// runtime.hip -- host side of libfpho_hip.so: device memory for patches, tile
+// plans, batches, the synchronous drop-in and the stream-ordered entry points.
+//
+// Ownership: a patch owns its HBM copies of the pixel and metadata arrays
+// (forcepho/patches/device_patch.py:268-393 holds the same arrays as pycuda
+// DeviceAllocations).  A plan (single patch or batch) owns its descriptors,
+// the image-Gaussian workspace and the tile partials; nothing is allocated
+// during an evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fpho.h"
+#include "fpho_internal.h"
+#include "kernels.h"
+
+using namespace fpho;
+
+// ------------------------------------------------------------------ errors
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fail(FPHO_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));    \
+    } while (0)
+
+// Device buffer with RAII release.
+template <typename T>
+struct DevVec {
+    T *ptr = nullptr;
+    int64_t n = 0;
+    DevVec() = default;
+    DevVec(const DevVec &) = delete;
+    DevVec &operator=(const DevVec &) = delete;
+    ~DevVec() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(int64_t count) {
+        release();
+        if (count <= 0) return hipSuccess;
+        hipError_t e = hipMalloc((void **)&ptr, sizeof(T) * (size_t)count);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    hipError_t upload(const T *host, int64_t count) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count <= 0) return e;
+        return hipMemcpy(ptr, host, sizeof(T) * (size_t)count, hipMemcpyHostToDevice);
+    }
+    void swap(DevVec &o) {
+        std::swap(ptr, o.ptr);
+        std::swap(n, o.n);
+    }
+};
+
+bool finite_all(const float *p, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+fpho_launch default_launch(bool batch) {
+    fpho_launch l;
+    l.pix_per_lane = 4;
+    l.waves_per_group = 4;
+    l.persistent = batch ? 1 : 0;
+    l.groups_per_cu = 8;
+    return l;
+}
+
+int check_launch(const fpho_launch &l) {
+    if (l.pix_per_lane != 1 && l.pix_per_lane != 2 && l.pix_per_lane != 4 &&
+        l.pix_per_lane != 8)
+        return fail(FPHO_E_ARG, "pix_per_lane must be 1, 2, 4 or 8 (got %d)", l.pix_per_lane);
+    if (l.waves_per_group < 1 || l.waves_per_group > 4)
+        return fail(FPHO_E_ARG, "waves_per_group must be in 1..4 (got %d)", l.waves_per_group);
+    if (l.persistent != 0 && l.persistent != 1)
+        return fail(FPHO_E_ARG, "persistent must be 0 or 1");
+    if (l.groups_per_cu < 1 || l.groups_per_cu > 64)
+        return fail(FPHO_E_ARG, "groups_per_cu must be in 1..64");
+    return FPHO_OK;
+}
+
+int cu_count_cached() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- objects
+
+// Tile plan over one or more patches (single-patch drop-in or batch).
+struct Plan {
+    DevVec<ExpDesc> exps;
+    DevVec<TileDesc> tiles;
+    DevVec<int32_t> queue;
+    DevVec<OutDesc> outs;
+    DevVec<ImgGauss> gauss;
+    DevVec<float> gpart;
+    DevVec<double> cpart;
+    DevVec<int32_t> counter;
+    DevVec<SrcDesc> srcs;
+    DevVec<PatchOut> pouts;
+    int64_t n_exp = 0, n_tiles = 0, n_outs = 0, n_gauss = 0, n_srcs = 0, n_patches = 0;
+    int64_t n_pairs = 0, npix = 0;
+    std::vector<int64_t> src_off, chi2_off, dchi2_off, theta_off;  // per patch, +1
+    fpho_launch launch;
+    int reduce_fp64 = 1;
+    bool valid = false;
+};
+
+struct fpho_patch {
+    int device = 0;
+    int64_t npix = 0;
+    int32_t n_bands = 0, n_exp = 0, n_sources = 0, n_radii = 0;
+    int64_t n_psfgauss = 0;
+    std::vector<int16_t> band_start, band_N;
+    std::vector<int32_t> exposure_start, exposure_N, n_psf_per_source, psfgauss_start;
+    std::vector<float> G_host;
+    DevVec<float> data, ierr, xpix, ypix, residual;
+    DevVec<float> rad2, D, crpix, crval, CW;
+    DevVec<fpho_psf_gauss> psf;
+    bool return_residual = false;
+    uint64_t generation = 1;  // bumps on metadata change or data/residual swap
+    fpho_launch launch = default_launch(false);
+    Plan plan;
+    uint64_t plan_generation = 0;
+    // staging for the synchronous drop-in
+    DevVec<fpho_source> prop_dev;
+    DevVec<double> chi2_dev, dchi2_dev;
+    std::vector<double> dchi2_host;
+};
+
+struct fpho_batch {
+    std::vector<fpho_patch *> patches;
+    std::vector<uint64_t> generations;
+    Plan plan;
+};
+
+struct fpho_spline {
+    DevVec<double> coeffs, tx, ty, radii;
+    SplineView view;
+};
+
+// ------------------------------------------------------------ validation
+
+namespace {
+
+int validate_meta(const fpho_patch_desc *d, int32_t n_bands, int32_t n_exp) {
+    if (d->n_sources < 1 || d->n_sources > FPHO_MAX_NATIVE_SOURCES)
+        return fail(FPHO_E_LIMIT, "n_sources=%d outside 1..%d", d->n_sources,
+                    FPHO_MAX_NATIVE_SOURCES);
+    if (d->n_radii < 1 || d->n_radii > FPHO_MAXRADII)
+        return fail(FPHO_E_LIMIT, "n_radii=%d outside 1..%d", d->n_radii, FPHO_MAXRADII);
+    if (!d->rad2 || !d->D || !d->crpix || !d->crval || !d->CW || !d->G ||
+        !d->n_psf_per_source || !d->psfgauss || !d->psfgauss_start)
+        return fail(FPHO_E_ARG, "null metadata pointer");
+    const int64_t nes = (int64_t)n_exp * d->n_sources;
+    if (!finite_all(d->rad2, d->n_radii) || !finite_all(d->D, 4 * nes) ||
+        !finite_all(d->CW, 4 * nes) || !finite_all(d->crpix, 2 * nes) ||
+        !finite_all(d->crval, 2 * nes) || !finite_all(d->G, n_exp))
+        return fail(FPHO_E_VALUE, "non-finite astrometry or calibration value");
+    for (int b = 0; b < n_bands; ++b) {
+        const int np = d->n_psf_per_source[b];
+        if (np < 1 || np > 4096) return fail(FPHO_E_LIMIT, "n_psf_per_source[%d]=%d", b, np);
+        if ((int64_t)np * d->n_sources > (1 << 20))
+            return fail(FPHO_E_LIMIT, "too many image Gaussians in band %d", b);
+    }
+    for (int b = 0; b < n_bands; ++b) {
+        for (int e = d->band_start[b]; e < d->band_start[b] + d->band_N[b]; ++e) {
+            const int64_t s0 = d->psfgauss_start[e];
+            const int np = d->n_psf_per_source[b];
+            if (s0 < 0 || s0 + np > d->n_psfgauss)
+                return fail(FPHO_E_ARG, "psfgauss_start[%d]=%lld out of range", e,
+                            (long long)s0);
+            for (int p = 0; p < np; ++p) {
+                const fpho_psf_gauss &g = d->psfgauss[s0 + p];
+                if (g.sersic_radius_bin < 0 || g.sersic_radius_bin >= d->n_radii)
+                    return fail(FPHO_E_ARG, "sersic_radius_bin %d out of range",
+                                g.sersic_radius_bin);
+                const float v[6] = {g.amp, g.xcen, g.ycen, g.Cxx, g.Cyy, g.Cxy};
+                if (!finite_all(v, 6)) return fail(FPHO_E_VALUE, "non-finite PSF Gaussian");
+                if (!(g.Cxx > 0.0f && g.Cyy > 0.0f && g.Cxx * g.Cyy - g.Cxy * g.Cxy > 0.0f))
+                    return fail(FPHO_E_VALUE,
+                                "PSF covariance not positive definite (exposure %d)", e);
+            }
+        }
+    }
+    return FPHO_OK;
+}
+
+int validate_pixels(const fpho_patch_desc *d) {
+    if (d->npix < 1 || d->npix > ((int64_t)1 << 31) - 1)
+        return fail(FPHO_E_LIMIT, "npix=%lld outside 1..2^31-1", (long long)d->npix);
+    if (d->n_bands < 1 || d->n_bands > FPHO_MAXBANDS)
+        return fail(FPHO_E_LIMIT, "n_bands=%d outside 1..%d", d->n_bands, FPHO_MAXBANDS);
+    if (d->n_exp < 1) return fail(FPHO_E_ARG, "n_exp=%d", d->n_exp);
+    if (!d->data || !d->ierr || !d->xpix || !d->ypix || !d->band_start || !d->band_N ||
+        !d->exposure_start || !d->exposure_N)
+        return fail(FPHO_E_ARG, "null pixel or index pointer");
+    for (int b = 0; b < d->n_bands; ++b) {
+        if (d->band_start[b] < 0 || d->band_N[b] < 0 ||
+            d->band_start[b] + d->band_N[b] > d->n_exp)
+            return fail(FPHO_E_ARG, "band %d exposures [%d, +%d) outside 0..%d", b,
+                        d->band_start[b], d->band_N[b], d->n_exp);
+    }
+    for (int e = 0; e < d->n_exp; ++e) {
+        if (d->exposure_start[e] < 0 || d->exposure_N[e] < 0 ||
+            (int64_t)d->exposure_start[e] + d->exposure_N[e] > d->npix)
+            return fail(FPHO_E_ARG, "exposure %d pixels [%d, +%d) outside 0..%lld", e,
+                        d->exposure_start[e], d->exposure_N[e], (long long)d->npix);
+    }
+    if (!finite_all(d->data, d->npix) || !finite_all(d->ierr, d->npix) ||
+        !finite_all(d->xpix, d->npix) || !finite_all(d->ypix, d->npix))
+        return fail(FPHO_E_VALUE, "non-finite pixel data, ierr or coordinate");
+    for (int64_t i = 0; i < d->npix; ++i)
+        if (d->ierr[i] < 0.0f) return fail(FPHO_E_VALUE, "negative ierr at pixel %lld",
+                                           (long long)i);
+    return FPHO_OK;
+}
+
+int upload_meta(fpho_patch *p, const fpho_patch_desc *d) {
+    const int64_t nes = (int64_t)p->n_exp * d->n_sources;
+    p->n_sources = d->n_sources;
+    p->n_radii = d->n_radii;
+    p->n_psfgauss = d->n_psfgauss;
+    p->n_psf_per_source.assign(d->n_psf_per_source, d->n_psf_per_source + p->n_bands);
+    p->psfgauss_start.assign(d->psfgauss_start, d->psfgauss_start + p->n_exp);
+    p->G_host.assign(d->G, d->G + p->n_exp);
+    HIP_TRY(p->rad2.upload(d->rad2, d->n_radii));
+    HIP_TRY(p->D.upload(d->D, 4 * nes));
+    HIP_TRY(p->CW.upload(d->CW, 4 * nes));
+    HIP_TRY(p->crpix.upload(d->crpix, 2 * nes));
+    HIP_TRY(p->crval.upload(d->crval, 2 * nes));
+    HIP_TRY(p->psf.upload(d->psfgauss, d->n_psfgauss));
+    ++p->generation;
+    return FPHO_OK;
+}
+
+// ------------------------------------------------------------- planning
+
+struct HostPlan {
+    std::vector<ExpDesc> exps;
+    std::vector<TileDesc> tiles;
+    std::vector<int64_t> cost;
+    std::vector<OutDesc> outs;
+    std::vector<SrcDesc> srcs;
+    std::vector<PatchOut> pouts;
+    int64_t n_gauss = 0, n_part = 0, n_pairs = 0, npix = 0;
+};
+
+// Append one patch to a host plan.  Tiles: each exposure is cut into tiles of
+// 64*K consecutive pixels (superpixel order keeps them compact on the sky).
+void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
+                int64_t dchi2_base, int64_t theta_base, int32_t patch_index, HostPlan &hp) {
+    const int tile_pix = kWave * K;
+    const int ns = p->n_sources;
+    for (int b = 0; b < p->n_bands; ++b) {
+        OutDesc od;
+        od.tile0 = (int64_t)hp.tiles.size();
+        od.part0 = hp.n_part;
+        od.chi2_out = chi2_base + b;
+        od.dchi2_out = dchi2_base + (int64_t)b * ns * kNParams;
+        od.n_src = ns;
+        const int np = p->n_psf_per_source[b];
+        for (int e = p->band_start[b]; e < p->band_start[b] + p->band_N[b]; ++e) {
+            ExpDesc ed;
+            ed.data = p->data.ptr;
+            ed.ierr = p->ierr.ptr;
+            ed.xpix = p->xpix.ptr;
+            ed.ypix = p->ypix.ptr;
+            ed.residual = p->return_residual ? p->residual.ptr : nullptr;
+            ed.rad2 = p->rad2.ptr;
+            ed.D = p->D.ptr + (int64_t)4 * ns * e;
+            ed.CW = p->CW.ptr + (int64_t)4 * ns * e;
+            ed.crpix = p->crpix.ptr + (int64_t)2 * ns * e;
+            ed.crval = p->crval.ptr + (int64_t)2 * ns * e;
+            ed.psf = p->psf.ptr + p->psfgauss_start[e];
+            ed.gauss_off = hp.n_gauss;
+            ed.src_off = (int32_t)src_base;
+            ed.band = b;
+            ed.n_src = ns;
+            ed.n_psf = np;
+            ed.n_radii = p->n_radii;
+            ed.G = p->G_host[e];
+            const int32_t ei = (int32_t)hp.exps.size();
+            hp.exps.push_back(ed);
+            hp.n_gauss += (int64_t)ns * np;
+            const int32_t n_e = p->exposure_N[e];
+            hp.npix += n_e;
+            hp.n_pairs += (int64_t)n_e * ns * np;
+            for (int32_t off = 0; off < n_e; off += tile_pix) {
+                TileDesc t;
+                t.exp = ei;
+                t.npix = std::min(tile_pix, n_e - off);
+                t.pix0 = p->exposure_start[e] + off;
+                t.pad = 0;
+                t.part_off = hp.n_part;
+                hp.n_part += (int64_t)ns * kNParams;
+                hp.tiles.push_back(t);
+                // cost model: both passes visit every (pixel, Gaussian) pair
+                hp.cost.push_back((int64_t)ns * np * t.npix);
+            }
+        }
+        od.ntiles = (int32_t)((int64_t)hp.tiles.size() - od.tile0);
+        hp.outs.push_back(od);
+    }
+    int64_t theta = theta_base;
+    for (int s = 0; s < ns; ++s) {
+        SrcDesc sd;
+        sd.theta_off = theta;
+        sd.dchi2_base = dchi2_base + (int64_t)s * kNParams;
+        sd.n_bands = p->n_bands;
+        sd.src_stride = ns * kNParams;
+        sd.proposal = (int32_t)(src_base + s);
+        sd.pad = 0;
+        hp.srcs.push_back(sd);
+        theta += p->n_bands + 6;
+    }
+    PatchOut po;
+    po.chi2_off = chi2_base;
+    po.n_bands = p->n_bands;
+    po.pad = patch_index;
+    hp.pouts.push_back(po);
+}
+
+int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &launch,
+               Plan &plan) {
+    HostPlan hp;
+    plan.src_off.assign(1, 0);
+    plan.chi2_off.assign(1, 0);
+    plan.dchi2_off.assign(1, 0);
+    plan.theta_off.assign(1, 0);
+    for (size_t i = 0; i < patches.size(); ++i) {
+        const fpho_patch *p = patches[i];
+        plan_patch(p, launch.pix_per_lane, plan.src_off.back(), plan.chi2_off.back(),
+                   plan.dchi2_off.back(), plan.theta_off.back(), (int32_t)i, hp);
+        plan.src_off.push_back(plan.src_off.back() + p->n_sources);
+        plan.chi2_off.push_back(plan.chi2_off.back() + p->n_bands);
+        plan.dchi2_off.push_back(plan.dchi2_off.back() +
+                                 (int64_t)p->n_bands * p->n_sources * kNParams);
+        plan.theta_off.push_back(plan.theta_off.back() +
+                                 (int64_t)p->n_sources * (p->n_bands + 6));
+    }
+    if ((int64_t)hp.tiles.size() > INT32_MAX)
+        return fail(FPHO_E_LIMIT, "too many tiles (%zu)", hp.tiles.size());
+    // Queue order: most expensive tiles first (longest-processing-time first);
+    // ties keep canonical order, so tiles of one exposure stay adjacent.
+    std::vector<int32_t> queue(hp.tiles.size());
+    std::iota(queue.begin(), queue.end(), 0);
+    std::stable_sort(queue.begin(), queue.end(),
+                     [&](int32_t a, int32_t b) { return hp.cost[a] > hp.cost[b]; });
+
+    HIP_TRY(plan.exps.upload(hp.exps.data(), (int64_t)hp.exps.size()));
+    HIP_TRY(plan.tiles.upload(hp.tiles.data(), (int64_t)hp.tiles.size()));
+    HIP_TRY(plan.queue.upload(queue.data(), (int64_t)queue.size()));
+    HIP_TRY(plan.outs.upload(hp.outs.data(), (int64_t)hp.outs.size()));
+    HIP_TRY(plan.srcs.upload(hp.srcs.data(), (int64_t)hp.srcs.size()));
+    HIP_TRY(plan.pouts.upload(hp.pouts.data(), (int64_t)hp.pouts.size()));
+    HIP_TRY(plan.gauss.alloc(hp.n_gauss));
+    HIP_TRY(plan.gpart.alloc(std::max<int64_t>(hp.n_part, 1)));
+    HIP_TRY(plan.cpart.alloc(std::max<int64_t>((int64_t)hp.tiles.size(), 1)));
+    HIP_TRY(plan.counter.alloc(1));
+    HIP_TRY(hipMemset(plan.counter.ptr, 0, sizeof(int32_t)));
+    plan.n_exp = (int64_t)hp.exps.size();
+    plan.n_tiles = (int64_t)hp.tiles.size();
+    plan.n_outs = (int64_t)hp.outs.size();
+    plan.n_gauss = hp.n_gauss;
+    plan.n_srcs = (int64_t)hp.srcs.size();
+    plan.n_patches = (int64_t)patches.size();
+    plan.n_pairs = hp.n_pairs;
+    plan.npix = hp.npix;
+    plan.launch = launch;
+    plan.valid = true;
+    return FPHO_OK;
+}
+
+int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
+             hipStream_t stream) {
+    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, stream));
+    EvalArgs a;
+    a.tiles = plan.tiles.ptr;
+    a.queue = plan.queue.ptr;
+    a.n_tiles = plan.n_tiles;
+    a.counter = plan.counter.ptr;
+    a.exps = plan.exps.ptr;
+    a.gauss = plan.gauss.ptr;
+    a.gpart = plan.gpart.ptr;
+    a.cpart = plan.cpart.ptr;
+    a.pix_per_lane = plan.launch.pix_per_lane;
+    a.waves_per_group = plan.launch.waves_per_group;
+    a.persistent = plan.launch.persistent;
+    const int64_t groups_needed = (plan.n_tiles + a.waves_per_group - 1) / a.waves_per_group;
+    a.grid = std::min<int64_t>(groups_needed,
+                               (int64_t)plan.launch.groups_per_cu * cu_count_cached());
+    if (a.grid < 1) a.grid = 1;
+    HIP_TRY(launch_eval(a, stream));
+    HIP_TRY(launch_reduce(plan.outs.ptr, plan.n_outs, plan.gpart.ptr, plan.cpart.ptr, chi2,
+                          dchi2, plan.reduce_fp64, stream));
+    return FPHO_OK;
+}
+
+int ensure_patch_plan(fpho_patch *p) {
+    if (p->plan.valid && p->plan_generation == p->generation) return FPHO_OK;
+    std::vector<fpho_patch *> one{p};
+    const int rc = build_plan(one, p->launch, p->plan);
+    if (rc) return rc;
+    p->plan_generation = p->generation;
+    return FPHO_OK;
+}
+
+int ensure_batch_plan(fpho_batch *b) {
+    bool stale = !b->plan.valid;
+    for (size_t i = 0; i < b->patches.size() && !stale; ++i)
+        stale = b->generations[i] != b->patches[i]->generation;
+    if (!stale) return FPHO_OK;
+    const fpho_launch l = b->plan.valid ? b->plan.launch : default_launch(true);
+    const int fp64 = b->plan.reduce_fp64;
+    const int rc = build_plan(b->patches, l, b->plan);
+    if (rc) return rc;
+    b->plan.reduce_fp64 = fp64;
+    for (size_t i = 0; i < b->patches.size(); ++i) b->generations[i] = b->patches[i]->generation;
+    return FPHO_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char *fpho_last_error(void) { return g_err.c_str(); }
+
+int fpho_version(void) { return 1; }
+
+int fpho_init(int device) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(FPHO_E_ARG, "device %d outside 0..%d", device, n - 1);
+    HIP_TRY(hipSetDevice(device));
+    return FPHO_OK;
+}
+
+int fpho_device_cu_count(int *n_cu) {
+    if (!n_cu) return fail(FPHO_E_ARG, "null n_cu");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    return FPHO_OK;
+}
+
+int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch **out) {
+    if (!d || !out) return fail(FPHO_E_ARG, "null argument");
+    *out = nullptr;
+    int rc = validate_pixels(d);
+    if (rc) return rc;
+    rc = validate_meta(d, d->n_bands, d->n_exp);
+    if (rc) return rc;
+    fpho_patch *p = new (std::nothrow) fpho_patch();
+    if (!p) return fail(FPHO_E_LIMIT, "out of host memory");
+    (void)hipGetDevice(&p->device);
+    p->npix = d->npix;
+    p->n_bands = d->n_bands;
+    p->n_exp = d->n_exp;
+    p->band_start.assign(d->band_start, d->band_start + d->n_bands);
+    p->band_N.assign(d->band_N, d->band_N + d->n_bands);
+    p->exposure_start.assign(d->exposure_start, d->exposure_start + d->n_exp);
+    p->exposure_N.assign(d->exposure_N, d->exposure_N + d->n_exp);
+    p->return_residual = return_residual != 0;
+    hipError_t e = hipSuccess;
+    if ((e = p->data.upload(d->data, d->npix)) != hipSuccess ||
+        (e = p->ierr.upload(d->ierr, d->npix)) != hipSuccess ||
+        (e = p->xpix.upload(d->xpix, d->npix)) != hipSuccess ||
+        (e = p->ypix.upload(d->ypix, d->npix)) != hipSuccess) {
+        delete p;
+        return fail(FPHO_E_HIP, "pixel upload failed: %s", hipGetErrorString(e));
+    }
+    if (p->return_residual) {
+        if ((e = p->residual.alloc(d->npix)) != hipSuccess ||
+            (e = hipMemset(p->residual.ptr, 0, sizeof(float) * d->npix)) != hipSuccess) {
+            delete p;
+            return fail(FPHO_E_HIP, "residual allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    rc = upload_meta(p, d);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return FPHO_OK;
+}
+
+int fpho_patch_set_meta(fpho_patch *p, const fpho_patch_desc *d) {
+    if (!p || !d) return fail(FPHO_E_ARG, "null argument");
+    fpho_patch_desc m = *d;
+    m.n_bands = p->n_bands;
+    m.n_exp = p->n_exp;
+    m.band_start = p->band_start.data();
+    m.band_N = p->band_N.data();
+    const int rc = validate_meta(&m, p->n_bands, p->n_exp);
+    if (rc) return rc;
+    return upload_meta(p, d);
+}
+
+int fpho_patch_swap_data_residual(fpho_patch *p) {
+    if (!p) return fail(FPHO_E_ARG, "null patch");
+    if (!p->return_residual)
+        return fail(FPHO_E_STATE, "patch was created without a residual buffer");
+    p->data.swap(p->residual);
+    ++p->generation;
+    return FPHO_OK;
+}
+
+int fpho_patch_read(fpho_patch *p, int which, float *host_out) {
+    if (!p || !host_out) return fail(FPHO_E_ARG, "null argument");
+    const DevVec<float> *src = nullptr;
+    switch (which) {
+        case 0: src = &p->data; break;
+        case 1: src = &p->ierr; break;
+        case 2: src = &p->xpix; break;
+        case 3: src = &p->ypix; break;
+        case 4: src = &p->residual; break;
+        default: return fail(FPHO_E_ARG, "unknown array %d", which);
+    }
+    if (!src->ptr) return fail(FPHO_E_STATE, "array %d is not allocated", which);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(host_out, src->ptr, sizeof(float) * p->npix, hipMemcpyDeviceToHost));
+    return FPHO_OK;
+}
+
+int fpho_patch_info(const fpho_patch *p, int32_t *n_bands, int32_t *n_sources, int64_t *npix,
+                    int64_t *n_pairs) {
+    if (!p) return fail(FPHO_E_ARG, "null patch");
+    if (n_bands) *n_bands = p->n_bands;
+    if (n_sources) *n_sources = p->n_sources;
+    if (npix) *npix = p->npix;
+    if (n_pairs) {
+        int64_t acc = 0;
+        for (int b = 0; b < p->n_bands; ++b)
+            for (int e = p->band_start[b]; e < p->band_start[b] + p->band_N[b]; ++e)
+                acc += (int64_t)p->exposure_N[e] * p->n_sources * p->n_psf_per_source[b];
+        *n_pairs = acc;
+    }
+    return FPHO_OK;
+}
+
+int fpho_patch_set_launch(fpho_patch *p, const fpho_launch *l) {
+    if (!p || !l) return fail(FPHO_E_ARG, "null argument");
+    const int rc = check_launch(*l);
+    if (rc) return rc;
+    p->launch = *l;
+    p->plan.valid = false;
+    return FPHO_OK;
+}
+
+int fpho_patch_free(fpho_patch *p) {
+    if (!p) return FPHO_OK;
+    (void)hipDeviceSynchronize();
+    delete p;
+    return FPHO_OK;
+}
+
+int fpho_evaluate(fpho_patch *p, const fpho_source *prop, double *chi2, float *dchi2) {
+    if (!p || !prop || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
+    for (int s = 0; s < p->n_sources; ++s) {
+        const fpho_source &src = prop[s];
+        if (!(src.q > 0.0f) || !std::isfinite(src.q) || !std::isfinite(src.pa) ||
+            !std::isfinite(src.ra) || !std::isfinite(src.dec))
+            return fail(FPHO_E_VALUE, "source %d: q must be > 0 and positions finite", s);
+        for (int b = 0; b < p->n_bands; ++b)
+            if (!std::isfinite(src.fluxes[b]))
+                return fail(FPHO_E_VALUE, "source %d: non-finite flux in band %d", s, b);
+        for (int m = 0; m < p->n_radii; ++m)
+            if (!std::isfinite(src.mixture_amplitudes[m]) ||
+                !std::isfinite(src.damplitude_drh[m]) ||
+                !std::isfinite(src.damplitude_dnsersic[m]))
+                return fail(FPHO_E_VALUE, "source %d: non-finite mixture amplitude", s);
+    }
+    int rc = ensure_patch_plan(p);
+    if (rc) return rc;
+    const int64_t nd = (int64_t)p->n_bands * p->n_sources * kNParams;
+    if (p->prop_dev.n < p->n_sources) HIP_TRY(p->prop_dev.alloc(p->n_sources));
+    if (p->chi2_dev.n < p->n_bands) HIP_TRY(p->chi2_dev.alloc(p->n_bands));
+    if (p->dchi2_dev.n < nd) HIP_TRY(p->dchi2_dev.alloc(nd));
+    HIP_TRY(hipMemcpy(p->prop_dev.ptr, prop, sizeof(fpho_source) * p->n_sources,
+                      hipMemcpyHostToDevice));
+    rc = run_plan(p->plan, p->prop_dev.ptr, p->chi2_dev.ptr, p->dchi2_dev.ptr, nullptr);
+    if (rc) return rc;
+    p->dchi2_host.resize(nd);
+    HIP_TRY(hipMemcpy(chi2, p->chi2_dev.ptr, sizeof(double) * p->n_bands,
+                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p->dchi2_host.data(), p->dchi2_dev.ptr, sizeof(double) * nd,
+                      hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nd; ++i) dchi2[i] = (float)p->dchi2_host[i];
+    return FPHO_OK;
+}
+
+int fpho_evaluate_async(fpho_patch *p, const fpho_source *prop_dev, double *chi2_dev,
+                        double *dchi2_dev, void *stream) {
+    if (!p || !prop_dev || !chi2_dev || !dchi2_dev) return fail(FPHO_E_ARG, "null argument");
+    int rc = ensure_patch_plan(p);
+    if (rc) return rc;
+    return run_plan(p->plan, prop_dev, chi2_dev, dchi2_dev, (hipStream_t)stream);
+}
+
+int fpho_batch_create(fpho_patch *const *patches, int n, fpho_batch **out) {
+    if (!patches || !out || n < 1) return fail(FPHO_E_ARG, "need at least one patch");
+    *out = nullptr;
+    fpho_batch *b = new (std::nothrow) fpho_batch();
+    if (!b) return fail(FPHO_E_LIMIT, "out of host memory");
+    for (int i = 0; i < n; ++i) {
+        if (!patches[i]) {
+            delete b;
+            return fail(FPHO_E_ARG, "patch %d is null", i);
+        }
+        b->patches.push_back(patches[i]);
+        b->generations.push_back(0);
+    }
+    const int rc = ensure_batch_plan(b);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return FPHO_OK;
+}
+
+int fpho_batch_layout(const fpho_batch *b, int64_t *src_off, int64_t *chi2_off,
+                      int64_t *dchi2_off, int64_t *theta_off) {
+    if (!b) return fail(FPHO_E_ARG, "null batch");
+    const size_t n = b->plan.src_off.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (src_off) src_off[i] = b->plan.src_off[i];
+        if (chi2_off) chi2_off[i] = b->plan.chi2_off[i];
+        if (dchi2_off) dchi2_off[i] = b->plan.dchi2_off[i];
+        if (theta_off) theta_off[i] = b->plan.theta_off[i];
+    }
+    return FPHO_OK;
+}
+
+int fpho_batch_info(const fpho_batch *b, int64_t *n_tiles, int64_t *n_pairs, int64_t *npix,
+                    int64_t *n_gauss) {
+    if (!b) return fail(FPHO_E_ARG, "null batch");
+    if (n_tiles) *n_tiles = b->plan.n_tiles;
+    if (n_pairs) *n_pairs = b->plan.n_pairs;
+    if (npix) *npix = b->plan.npix;
+    if (n_gauss) *n_gauss = b->plan.n_gauss;
+    return FPHO_OK;
+}
+
+int fpho_batch_set_launch(fpho_batch *b, const fpho_launch *l) {
+    if (!b || !l) return fail(FPHO_E_ARG, "null argument");
+    const int rc = check_launch(*l);
+    if (rc) return rc;
+    const int fp64 = b->plan.reduce_fp64;
+    const int r2 = build_plan(b->patches, *l, b->plan);
+    b->plan.reduce_fp64 = fp64;
+    return r2;
+}
+
+int fpho_batch_set_precision(fpho_batch *b, int reduce_fp64) {
+    if (!b) return fail(FPHO_E_ARG, "null batch");
+    b->plan.reduce_fp64 = reduce_fp64 ? 1 : 0;
+    return FPHO_OK;
+}
+
+int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *props, double *chi2,
+                              double *dchi2, void *stream) {
+    if (!b || !props || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
+    const int rc = ensure_batch_plan(b);
+    if (rc) return rc;
+    return run_plan(b->plan, props, chi2, dchi2, (hipStream_t)stream);
+}
+
+int fpho_batch_free(fpho_batch *b) {
+    if (!b) return FPHO_OK;
+    (void)hipDeviceSynchronize();
+    delete b;
+    return FPHO_OK;
+}
+
+int fpho_spline_create(const double *coeffs, int n_amp, const double *tx, int n_tx,
+                       const double *ty, int n_ty, const double *radii, double nsersic_min,
+                       fpho_spline **out) {
+    if (!coeffs || !tx || !ty || !radii || !out) return fail(FPHO_E_ARG, "null argument");
+    *out = nullptr;
+    if (n_amp < 1 || n_amp > FPHO_MAXRADII) return fail(FPHO_E_LIMIT, "n_amp=%d", n_amp);
+    if (n_tx < 8 || n_ty < 8) return fail(FPHO_E_ARG, "cubic splines need >= 8 knots");
+    for (int i = 1; i < n_tx; ++i)
+        if (!(tx[i] >= tx[i - 1])) return fail(FPHO_E_ARG, "tx not sorted");
+    for (int i = 1; i < n_ty; ++i)
+        if (!(ty[i] >= ty[i - 1])) return fail(FPHO_E_ARG, "ty not sorted");
+    fpho_spline *s = new (std::nothrow) fpho_spline();
+    if (!s) return fail(FPHO_E_LIMIT, "out of host memory");
+    const int64_t nc = (int64_t)n_amp * (n_tx - 4) * (n_ty - 4);
+    hipError_t e;
+    if ((e = s->coeffs.upload(coeffs, nc)) != hipSuccess ||
+        (e = s->tx.upload(tx, n_tx)) != hipSuccess ||
+        (e = s->ty.upload(ty, n_ty)) != hipSuccess ||
+        (e = s->radii.upload(radii, n_amp)) != hipSuccess) {
+        delete s;
+        return fail(FPHO_E_HIP, "spline upload failed: %s", hipGetErrorString(e));
+    }
+    s->view.coeffs = s->coeffs.ptr;
+    s->view.tx = s->tx.ptr;
+    s->view.ty = s->ty.ptr;
+    s->view.radii = s->radii.ptr;
+    s->view.n_tx = n_tx;
+    s->view.n_ty = n_ty;
+    s->view.n_amp = n_amp;
+    s->view.pad = 0;
+    s->view.nsersic_min = nsersic_min;
+    *out = s;
+    return FPHO_OK;
+}
+
+int fpho_spline_free(fpho_spline *s) {
+    if (!s) return FPHO_OK;
+    (void)hipDeviceSynchronize();
+    delete s;
+    return FPHO_OK;
+}
+
+int fpho_batch_theta_to_proposals_async(fpho_batch *b, const fpho_spline *s,
+                                        const double *theta, fpho_source *props,
+                                        void *stream) {
+    if (!b || !s || !theta || !props) return fail(FPHO_E_ARG, "null argument");
+    const int rc = ensure_batch_plan(b);
+    if (rc) return rc;
+    HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props,
+                         (hipStream_t)stream));
+    return FPHO_OK;
+}
+
+int fpho_batch_stack_grad_async(fpho_batch *b, const double *chi2, const double *dchi2,
+                                double *lnlike, double *grad, void *stream) {
+    if (!b || !chi2 || !dchi2 || !lnlike || !grad) return fail(FPHO_E_ARG, "null argument");
+    const int rc = ensure_batch_plan(b);
+    if (rc) return rc;
+    HIP_TRY(launch_stack(b->plan.srcs.ptr, b->plan.n_srcs, b->plan.pouts.ptr,
+                         b->plan.n_patches, chi2, dchi2, lnlike, grad, (hipStream_t)stream));
+    return FPHO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+"""Device mix-ins: move a packed patch to HBM and drive evaluations.
+
+``HIPPatchMixin`` is the drop-in for ``GPUPatchMixin``
+(forcepho/patches/device_patch.py:260-438): same method names and meaning
+(``send_to_device``, ``replace_device_meta_ptrs``,
+``send_patchstruct_to_device``, ``swap_on_device``, ``retrieve_array``,
+``free``, ``get_proposer``) and the shared ``DevicePatchMixin`` workflow
+(``prepare_model``, ``subtract_fixed``, ``design_matrix``, ``render``,
+lines 23-257).  Device memory belongs to a ``fpho_patch`` object of
+libfpho_hip.so; ``device_patch`` is its opaque handle.
+"""
+
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+from ..limits import psf_struct_dtype
+
+__all__ = ["DevicePatchMixin", "HIPPatchMixin"]
+
+
+class DevicePatchMixin:
+    """Device-independent workflow on top of the device primitives."""
+
+    def prepare_model(self, active=None, fixed=None, bounds=None, maxactive=15,
+                      model_kwargs={}):
+        """Subtract fixed sources, then wrap the patch in a FastPosterior
+        (forcepho/patches/device_patch.py:28-72).  ``bounds`` is a pair of
+        (lower, upper) arrays for a BoundedTransform."""
+        from ..model import FastPosterior, Transform
+        proposer, scene = self.subtract_fixed(fixed, active=active, maxactive=maxactive)
+        self.return_residual = False
+        q = scene.get_all_source_params().copy()
+        if bounds is None:
+            model = FastPosterior(proposer, scene=scene, patch=self,
+                                  transform=Transform(len(q)), **model_kwargs)
+        else:
+            lo, hi = bounds
+            model = FastPosterior(proposer, scene=scene, patch=self, lower=lo, upper=hi,
+                                  **model_kwargs)
+        return model, q
+
+    def subtract_fixed(self, fixed, active=None, maxactive=15, swap_local=False):
+        """Subtract blocks of fixed sources from the device-side data
+        (forcepho/patches/device_patch.py:113-183).  After the call the
+        device data hold the residual of all fixed sources and the device
+        metadata describe the last scene (``active`` if given)."""
+        scenes = []
+        if fixed is not None:
+            inds = np.arange(maxactive, len(fixed), maxactive)
+            scenes.extend([self.set_scene(block) for block in np.array_split(fixed, inds)])
+        if active is not None:
+            scenes.append(self.set_scene(active))
+        else:
+            assert swap_local
+            scenes.append(scenes[-1])
+        assert len(scenes) > 0, "No scenes to prepare!"
+        proposer = self.get_proposer()
+        self.return_residual = True
+        self.pack_meta(scenes[0])
+        self.send_to_device()
+        for i, scene in enumerate(scenes[1:]):
+            proposal = scenes[i].get_proposal()
+            proposer.evaluate_proposal(proposal, patch=self)
+            self.pack_meta(scene)
+            self.swap_on_device()
+        return proposer, scenes[-1]
+
+    def design_matrix(self, active=None, proposer=None):
+        """Per-source unit-flux models per band and the fixed-subtracted data
+        (forcepho/patches/device_patch.py:185-238)."""
+        y = self.retrieve_array("data")
+        ys = np.split(y, np.cumsum(self.band_N_pix[:-1]))
+        Xes = [np.zeros((len(active), n)) for n in self.band_N_pix]
+        scenes = [self.set_scene(np.atleast_1d(a)) for a in active]
+        proposer = proposer or self.get_proposer()
+        self.return_residual = True
+        for i, scene in enumerate(scenes):
+            self.pack_meta(scene)
+            self.replace_device_meta_ptrs()
+            self.send_patchstruct_to_device()
+            proposal = scene.get_proposal()
+            _, _, residual = proposer.evaluate_proposal(proposal, patch=self, unpack=False)
+            model = y - residual
+            split = np.split(model, np.cumsum(self.band_N_pix[:-1]))
+            for j, b in enumerate(self.bandlist):
+                Xes[j][i, :] = split[j] / active[i][b]
+        return Xes, ys
+
+    def render(self, catalog):
+        """Model images of ``catalog`` (minus sign of the residual against
+        zero data is the model; forcepho/patches/device_patch.py:240-254)."""
+        proposer = self.get_proposer()
+        scene = self.set_scene(catalog)
+        self.pack_meta(scene)
+        self.replace_device_meta_ptrs()
+        self.send_patchstruct_to_device()
+        proposal = scene.get_proposal()
+        self.return_residual = True
+        _, _, residuals = proposer.evaluate_proposal(proposal, patch=self, unpack=True)
+        return [-r for r in residuals]
+
+
+class HIPPatchMixin(DevicePatchMixin):
+    """Patch data in MI355X HBM through libfpho_hip.so."""
+
+    device = 0
+
+    def get_proposer(self):
+        from ..proposal import HIPProposer
+        return HIPProposer()
+
+    # -- descriptors -------------------------------------------------------
+    def _desc(self):
+        """fpho_patch_desc over this patch's host arrays (kept alive in
+        ``self._desc_keep`` until the call returns)."""
+        keep = {}
+
+        def arr(name, dtype, ctype):
+            a = np.ascontiguousarray(getattr(self, name), dtype=dtype).reshape(-1)
+            keep[name] = a
+            return _lib.fptr(a, ctype)
+
+        d = _lib.PatchDesc()
+        d.npix = int(self.npix)
+        d.data = arr("data", np.float32, ctypes.c_float)
+        d.ierr = arr("ierr", np.float32, ctypes.c_float)
+        d.xpix = arr("xpix", np.float32, ctypes.c_float)
+        d.ypix = arr("ypix", np.float32, ctypes.c_float)
+        d.n_bands = int(len(self.band_N))
+        d.n_exp = int(len(self.exposure_N))
+        d.band_start = arr("band_start", np.int16, ctypes.c_int16)
+        d.band_N = arr("band_N", np.int16, ctypes.c_int16)
+        d.exposure_start = arr("exposure_start", np.int32, ctypes.c_int32)
+        d.exposure_N = arr("exposure_N", np.int32, ctypes.c_int32)
+        d.n_sources = int(self.n_sources)
+        d.n_radii = int(self.n_radii)
+        d.rad2 = arr("rad2", np.float32, ctypes.c_float)
+        d.D = arr("D", np.float32, ctypes.c_float)
+        d.crpix = arr("crpix", np.float32, ctypes.c_float)
+        d.crval = arr("crval", np.float32, ctypes.c_float)
+        d.CW = arr("CW", np.float32, ctypes.c_float)
+        d.G = arr("G", np.float32, ctypes.c_float)
+        d.n_psf_per_source = arr("n_psf_per_source", np.int32, ctypes.c_int32)
+        psf = np.ascontiguousarray(self.psfgauss)
+        if psf.dtype != psf_struct_dtype:
+            psf = np.ascontiguousarray(psf.view(psf_struct_dtype) if
+                                       psf.dtype.itemsize == 28 else psf.astype(psf_struct_dtype))
+        keep["psfgauss"] = psf
+        d.psfgauss = ctypes.c_void_p(psf.ctypes.data)
+        d.n_psfgauss = int(len(psf))
+        d.psfgauss_start = arr("psfgauss_start", np.int32, ctypes.c_int32)
+        self._desc_keep = keep
+        return d
+
+    # -- device primitives -------------------------------------------------
+    def send_to_device(self):
+        """Copy all patch arrays to HBM (forcepho/patches/device_patch.py:
+        268-300); allocates the residual if ``return_residual``."""
+        _lib.init(self.device)
+        self.free()
+        handle = ctypes.c_void_p()
+        d = self._desc()
+        _lib.check(_lib.lib().fpho_patch_create(ctypes.byref(d), int(bool(self.return_residual)),
+                                                ctypes.byref(handle)))
+        self._desc_keep = None
+        self.device_patch = handle
+        self.device_ptrs = {"residual": bool(self.return_residual)}
+        self._has_residual = bool(self.return_residual)
+        self._dirty_data = False
+        return self.device_patch
+
+    def replace_device_meta_ptrs(self):
+        """Upload the currently packed metadata (device_patch.py:302-311)."""
+        d = self._desc()
+        _lib.check(_lib.lib().fpho_patch_set_meta(self.device_patch, ctypes.byref(d)))
+        self._desc_keep = None
+
+    def send_patchstruct_to_device(self):
+        """Nothing to do: the library rebuilds its descriptors lazily when the
+        metadata or the data/residual assignment changes."""
+        return self.device_patch
+
+    def swap_on_device(self):
+        """Upload new metadata and make the residual the new data
+        (device_patch.py:346-367)."""
+        assert getattr(self, "_has_residual", False), \
+            "Must instantiate the Patch with `return_residual=True`"
+        self.replace_device_meta_ptrs()
+        _lib.check(_lib.lib().fpho_patch_swap_data_residual(self.device_patch))
+        self._dirty_data = True
+        return self.device_patch
+
+    def retrieve_array(self, arrname="residual"):
+        """Copy one device pixel array to host (device_patch.py:369-375)."""
+        out = np.empty(self.npix, dtype=np.float32)
+        _lib.check(_lib.lib().fpho_patch_read(self.device_patch, _lib.ARRAY_IDS[arrname],
+                                              _lib.fptr(out, ctypes.c_float)))
+        return out
+
+    def set_launch(self, pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8):
+        l = _lib.Launch(pix_per_lane, waves_per_group, persistent, groups_per_cu)
+        _lib.check(_lib.lib().fpho_patch_set_launch(self.device_patch, ctypes.byref(l)))
+
+    def free(self):
+        handle = getattr(self, "device_patch", None)
+        if handle:
+            _lib.lib().fpho_patch_free(handle)
+        self.device_patch = None
+        self.device_ptrs = {}

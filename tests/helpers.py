@@ -1,0 +1,41 @@
+"""Shared builders and tolerance checks for the tests (test code only)."""
+import numpy as np
+
+import oracle
+from forcepho_amd import synth
+
+# Stated tolerances: chi2 relative error <= 1e-6 against the FP64 oracle;
+# each gradient component relative error <= 1e-4 or absolute error
+# <= 1e-6 * max |gradient| (of that band).
+CHI2_RTOL = 1e-6
+GRAD_RTOL = 1e-4
+GRAD_ATOL_FRAC = 1e-6
+
+
+def oracle_render(patch, proposal):
+    return oracle.evaluate(patch, proposal, model=True)[2]
+
+
+def make_config_patch(config, patch_id=0, return_residual=False):
+    spec = synth.config_spec(config)
+    return synth.make_patch(spec, config=config, patch_id=patch_id,
+                            return_residual=return_residual, render=oracle_render)
+
+
+def proposal_for(scene, theta):
+    scene.set_all_source_params(theta)
+    return scene.get_proposal().copy()
+
+
+def grad_mismatch(d, d_ref):
+    """Boolean mask of gradient components outside the stated tolerance."""
+    d = np.asarray(d, dtype=np.float64)
+    d_ref = np.asarray(d_ref, dtype=np.float64)
+    scale = np.abs(d_ref).reshape(d_ref.shape[0], -1).max(axis=1)
+    atol = GRAD_ATOL_FRAC * scale[:, None, None]
+    err = np.abs(d - d_ref)
+    return (err > GRAD_RTOL * np.abs(d_ref)) & (err > atol)
+
+
+def chi2_relerr(c, c_ref):
+    return np.abs(np.asarray(c) - c_ref) / np.abs(c_ref)

@@ -1,0 +1,83 @@
+"""Parity of the HIP path against the FP64 oracle (and the compiled
+reference CPU kernel where it is built) on configurations 1 and 2."""
+import numpy as np
+import pytest
+
+import oracle
+from forcepho_amd import synth
+from helpers import (CHI2_RTOL, chi2_relerr, grad_mismatch, make_config_patch,
+                     proposal_for)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1(hip):
+    patch, scene, theta = make_config_patch(1, return_residual=True)
+    patch.send_to_device()
+    yield patch, scene, theta
+    patch.free()
+
+
+@pytest.fixture(scope="module")
+def c2(hip):
+    patch, scene, theta = make_config_patch(2)
+    patch.send_to_device()
+    yield patch, scene, theta
+    patch.free()
+
+
+def _check(patch, proposal, label):
+    chi2, dchi2 = patch.get_proposer().evaluate_proposal(proposal, patch=patch)[:2]
+    c_ref, d_ref = oracle.evaluate(patch, proposal)
+    rel = chi2_relerr(chi2, c_ref)
+    bad = grad_mismatch(dchi2, d_ref)
+    print(f"{label}: chi2 rel err max {rel.max():.3e}; grad mismatches {bad.sum()}/{bad.size}",
+          flush=True)
+    assert rel.max() <= CHI2_RTOL, (label, rel)
+    assert not bad.any(), (label, np.argwhere(bad)[:5])
+    return chi2, dchi2
+
+
+def test_config1_truth_and_proposals(c1):
+    patch, scene, theta = c1
+    props = synth.make_proposals(theta, 1, 5, config=1)
+    _check(patch, proposal_for(scene, theta), "c1 truth")
+    for k, t in enumerate(props):
+        _check(patch, proposal_for(scene, t), f"c1 proposal {k}")
+
+
+def test_config1_residual(c1):
+    patch, scene, theta = c1
+    prop = proposal_for(scene, theta)
+    chi2, dchi2, res = patch.get_proposer().evaluate_proposal(prop, patch=patch, unpack=False)
+    _, _, r_ref = oracle.evaluate(patch, prop, residual=True)
+    scale = np.abs(patch.data).max()
+    assert np.abs(res - r_ref).max() <= 2e-6 * scale
+
+
+def test_config1_against_reference_kernel(c1):
+    if oracle.reference_module() is None:
+        pytest.skip("compiled reference kernel not present")
+    patch, scene, theta = c1
+    prop = proposal_for(scene, theta)
+    chi2, dchi2 = patch.get_proposer().evaluate_proposal(prop, patch=patch)[:2]
+    c_r, d_r = oracle.ReferenceKernel(patch).evaluate(prop)
+    assert chi2_relerr(chi2, c_r).max() <= CHI2_RTOL
+    assert not grad_mismatch(dchi2, d_r).any()
+
+
+def test_config2_proposals(c2):
+    patch, scene, theta = c2
+    props = synth.make_proposals(theta, 8, 3, config=2)
+    _check(patch, proposal_for(scene, theta), "c2 truth")
+    for k, t in enumerate(props):
+        _check(patch, proposal_for(scene, t), f"c2 proposal {k}")
+
+
+def test_determinism(c2):
+    patch, scene, theta = c2
+    prop = proposal_for(scene, theta)
+    a = patch.get_proposer().evaluate_proposal(prop, patch=patch)
+    b = patch.get_proposer().evaluate_proposal(prop, patch=patch)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
