@@ -24,16 +24,19 @@ c_i64_p = ctypes.POINTER(ctypes.c_int64)
 EXPORTED = [
     "fpho_last_error", "fpho_version", "fpho_init", "fpho_device_cu_count",
     "fpho_patch_create", "fpho_patch_set_meta", "fpho_patch_swap_data_residual",
-    "fpho_patch_read", "fpho_patch_info", "fpho_patch_set_launch", "fpho_patch_free",
+    "fpho_patch_read", "fpho_patch_info", "fpho_patch_set_launch",
+    "fpho_patch_set_precision", "fpho_patch_free",
     "fpho_evaluate", "fpho_evaluate_async",
     "fpho_batch_create", "fpho_batch_layout", "fpho_batch_info",
     "fpho_batch_set_launch", "fpho_batch_set_precision",
-    "fpho_batch_evaluate_async", "fpho_batch_free",
+    "fpho_batch_evaluate_async", "fpho_batch_set_timing", "fpho_batch_last_timing",
+    "fpho_batch_free",
     "fpho_spline_create", "fpho_spline_free",
     "fpho_batch_theta_to_proposals_async", "fpho_batch_stack_grad_async",
 ]
 
 ARRAY_IDS = {"data": 0, "ierr": 1, "xpix": 2, "ypix": 3, "residual": 4}
+PRECISION = {"fp32": 0, "fp64_reduce": 1, "mixed": 2}
 
 
 class FphoError(RuntimeError):
@@ -82,6 +85,7 @@ def _declare(lib):
         "fpho_patch_read": ([P, c_int, c_float_p], c_int),
         "fpho_patch_info": ([P, c_i32_p, c_i32_p, c_i64_p, c_i64_p], c_int),
         "fpho_patch_set_launch": ([P, ctypes.POINTER(Launch)], c_int),
+        "fpho_patch_set_precision": ([P, c_int], c_int),
         "fpho_patch_free": ([P], c_int),
         "fpho_evaluate": ([P, P, c_double_p, c_float_p], c_int),
         "fpho_evaluate_async": ([P, P, P, P, P], c_int),
@@ -91,6 +95,8 @@ def _declare(lib):
         "fpho_batch_set_launch": ([P, ctypes.POINTER(Launch)], c_int),
         "fpho_batch_set_precision": ([P, c_int], c_int),
         "fpho_batch_evaluate_async": ([P, P, P, P, P], c_int),
+        "fpho_batch_set_timing": ([P, c_int], c_int),
+        "fpho_batch_last_timing": ([P, c_float_p], c_int),
         "fpho_batch_free": ([P], c_int),
         "fpho_spline_create": ([c_double_p, c_int, c_double_p, c_int, c_double_p, c_int,
                                 c_double_p, ctypes.c_double, ctypes.POINTER(P)], c_int),

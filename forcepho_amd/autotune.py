@@ -1,0 +1,89 @@
+"""Launch-shape autotuner for the pixel kernel.
+
+Candidates are (pixels per lane K, waves per workgroup W, persistent or
+grid launch, workgroups per CU for the persistent grid).  Each candidate is
+timed with HIP events on the evaluation stream over a few repetitions of a
+representative batch; the fastest is stored in a JSON cache keyed by device
+name and workload kind, which ``load_or_default`` reads at start-up.  Results
+do not depend on the persistent/grid choice or on W (same tiles, same fixed
+reduction order); K changes the tiling and therefore only the rounding of
+the FP32 partial sums.
+"""
+
+import itertools
+import json
+import os
+
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
+DEFAULT = {"batch": dict(pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8),
+           "patch": dict(pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8)}
+
+__all__ = ["candidates", "tune", "load_or_default", "save"]
+
+
+def candidates(kind="batch"):
+    out = []
+    for K, W in itertools.product((1, 2, 4, 8), (1, 2, 4)):
+        if kind == "batch":
+            for G in (4, 8, 16):
+                out.append(dict(pix_per_lane=K, waves_per_group=W, persistent=1,
+                                groups_per_cu=G))
+        out.append(dict(pix_per_lane=K, waves_per_group=W, persistent=0, groups_per_cu=8))
+    return out
+
+
+def _device_name():
+    try:
+        import torch
+        return torch.cuda.get_device_name(0)
+    except Exception:
+        return "unknown"
+
+
+def tune(batch, run, cands=None, reps=3, log=None):
+    """Time ``run()`` (one evaluation on torch's current stream) for every
+    candidate launch of ``batch``; return (best, table)."""
+    import torch
+    cands = cands or candidates("batch")
+    table = []
+    for c in cands:
+        batch.set_launch(**c)
+        run()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            run()
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / reps
+        table.append((ms, c))
+        if log:
+            log(f"  {c}: {ms:.3f} ms")
+    table.sort(key=lambda t: t[0])
+    batch.set_launch(**table[0][1])
+    return table[0][1], table
+
+
+def save(kind, launch, ms=None):
+    data = {}
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            data = json.load(f)
+    data.setdefault(_device_name(), {})[kind] = dict(launch=launch, ms=ms)
+    with open(CACHE, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def load_or_default(kind="batch"):
+    """Cached best launch for this device and workload kind, else default."""
+    if os.path.exists(CACHE):
+        try:
+            with open(CACHE) as f:
+                data = json.load(f)
+            entry = data.get(_device_name(), {}).get(kind)
+            if entry:
+                return dict(entry["launch"])
+        except (OSError, ValueError, KeyError):
+            pass
+    return dict(DEFAULT[kind])

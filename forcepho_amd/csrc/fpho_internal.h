@@ -100,26 +100,51 @@ struct alignas(8) PatchOut {
     int32_t pad;
 };
 
+// FP64 copy of an image Gaussian's value parameters, used to re-evaluate the
+// few (pixel, Gaussian) terms that are bright relative to the pixel noise.
+struct alignas(16) ImgGauss64 {
+    double xcen, ycen, fxx, fyy, fxy, amp, pad0, pad1;
+};
+static_assert(sizeof(ImgGauss64) == 64, "ImgGauss64 must be 64 bytes");
+
+// Precision modes of the pixel pass and reductions.
+enum Precision : int32_t {
+    kPrecFP32 = 0,        // FP32 residual, FP32 cross-tile sums (ablation)
+    kPrecFP64Reduce = 1,  // FP32 residual (as the reference kernel), FP64 sums
+    kPrecMixed = 2,       // FP64 residual accumulation, bright terms in FP64,
+                          // FP64 sums (default)
+};
+// Terms brighter than kBrightSigma * sigma_pixel are evaluated in FP64.
+constexpr float kBrightSigma = 0.1f;
+
 // ------------------------------------------------------------ 2x2 algebra
 // Row-major [[a, b], [c, d]]; the order of forcepho/src/matrix22.cc.
+template <typename T>
 struct M22 {
-    float a, b, c, d;
+    T a, b, c, d;
 };
-FPHO_HD inline M22 mul(const M22 &x, const M22 &y) {
+template <typename T>
+FPHO_HD inline M22<T> mul(const M22<T> &x, const M22<T> &y) {
     return {x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d,
             x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
 }
-FPHO_HD inline M22 transpose(const M22 &x) { return {x.a, x.c, x.b, x.d}; }
-FPHO_HD inline M22 add(const M22 &x, const M22 &y) {
+template <typename T>
+FPHO_HD inline M22<T> transpose(const M22<T> &x) { return {x.a, x.c, x.b, x.d}; }
+template <typename T>
+FPHO_HD inline M22<T> add(const M22<T> &x, const M22<T> &y) {
     return {x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d};
 }
-FPHO_HD inline M22 scale(const M22 &x, float s) {
+template <typename T>
+FPHO_HD inline M22<T> scale(const M22<T> &x, T s) {
     return {x.a * s, x.b * s, x.c * s, x.d * s};
 }
-FPHO_HD inline float det(const M22 &x) { return x.a * x.d - x.b * x.c; }
-FPHO_HD inline float trace(const M22 &x) { return x.a + x.d; }
-FPHO_HD inline M22 inverse(const M22 &x) {
-    const float id = 1.0f / det(x);
+template <typename T>
+FPHO_HD inline T det(const M22<T> &x) { return x.a * x.d - x.b * x.c; }
+template <typename T>
+FPHO_HD inline T trace(const M22<T> &x) { return x.a + x.d; }
+template <typename T>
+FPHO_HD inline M22<T> inverse(const M22<T> &x) {
+    const T id = T(1) / det(x);
     return {x.d * id, -x.b * id, -x.c * id, x.a * id};
 }
 

@@ -14,12 +14,14 @@ struct EvalArgs {
     int32_t *counter;      // persistent work-queue counter (device)
     const ExpDesc *exps;
     const ImgGauss *gauss;
+    const ImgGauss64 *gauss64;
     float *gpart;
     double *cpart;
     int pix_per_lane;
     int waves_per_group;
     int persistent;
     int64_t grid;          // persistent: number of workgroups
+    int precision;         // Precision mode
 };
 
 struct SplineView {
@@ -32,7 +34,7 @@ struct SplineView {
 };
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, hipStream_t stream);
+                          ImgGauss *out, ImgGauss64 *out64, hipStream_t stream);
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,
                          const double *cpart, double *chi2, double *dchi2, int fp64,

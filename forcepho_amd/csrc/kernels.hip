@@ -24,94 +24,112 @@ namespace fpho {
 
 // ------------------------------------------------------------------ convert
 
-// Build one image Gaussian.  Same mathematics as GetGaussianAndJacobian
-// (common_kernel.cc:175-240) with the source set-up of
-// compute_gaussians_kernel.cu:76-124; 2x2 algebra in FP32.
-__device__ __forceinline__ ImgGauss make_image_gaussian(const ExpDesc &e,
-                                                        const fpho_source &src,
-                                                        int s, int p) {
+// Build one image Gaussian in FP64: the source set-up of
+// compute_gaussians_kernel.cu:76-124 and the convolution and Jacobian of
+// GetGaussianAndJacobian (common_kernel.cc:175-240).  The per-Gaussian cost
+// is negligible next to the pixel pass, so it runs in double and is rounded
+// once into the FP32 record the pixel pass streams.
+__device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
+                                                    const fpho_source &src, int s,
+                                                    int p, ImgGauss &g,
+                                                    ImgGauss64 &g64) {
+    using M = M22<double>;
     const float *Dp = e.D + 4 * s;
     const float *CWp = e.CW + 4 * s;
-    const M22 D{Dp[0], Dp[1], Dp[2], Dp[3]};
-    const M22 CW{CWp[0], CWp[1], CWp[2], CWp[3]};
+    const M D{Dp[0], Dp[1], Dp[2], Dp[3]};
 
-    float sn, cs;
-    sincosf(src.pa, &sn, &cs);
-    const M22 R{cs, -sn, sn, cs};
-    const M22 dR{-sn, -cs, cs, -sn};
-    const float iq = 1.0f / src.q;
-    const M22 S{iq, 0.0f, 0.0f, src.q};
-    const M22 dS{-iq * iq, 0.0f, 0.0f, 1.0f};
+    double sn, cs;
+    sincos((double)src.pa, &sn, &cs);
+    const double q = src.q;
+    const M R{cs, -sn, sn, cs};
+    const M dR{-sn, -cs, cs, -sn};
+    const double iq = 1.0 / q;
+    const M S{iq, 0.0, 0.0, q};
+    const M dS{-iq * iq, 0.0, 0.0, 1.0};
 
-    const M22 DR = mul(D, R);
-    const M22 T = mul(DR, S);
-    const M22 dT_dq = mul(DR, dS);
-    const M22 dT_dpa = mul(mul(D, dR), S);
+    const M DR = mul(D, R);
+    const M T = mul(DR, S);
+    const M dT_dq = mul(DR, dS);
+    const M dT_dpa = mul(mul(D, dR), S);
 
     // source centre in pixels: CW (ra - crval, dec - crval) + crpix
-    const float s0 = src.ra - e.crval[2 * s + 0];
-    const float s1 = src.dec - e.crval[2 * s + 1];
-    const float xs = CWp[0] * s0 + CWp[1] * s1 + e.crpix[2 * s + 0];
-    const float ys = CWp[2] * s0 + CWp[3] * s1 + e.crpix[2 * s + 1];
+    const double s0 = (double)src.ra - (double)e.crval[2 * s + 0];
+    const double s1 = (double)src.dec - (double)e.crval[2 * s + 1];
+    const double xs = (double)CWp[0] * s0 + (double)CWp[1] * s1 + (double)e.crpix[2 * s + 0];
+    const double ys = (double)CWp[2] * s0 + (double)CWp[3] * s1 + (double)e.crpix[2 * s + 1];
 
     const fpho_psf_gauss pg = e.psf[p];
     const int bin = pg.sersic_radius_bin;
-    const float r2 = e.rad2[bin];
+    const double r2 = e.rad2[bin];
 
     // covariance = r2 T T^T + C_psf ; F = covariance^-1
-    const M22 Tt = transpose(T);
-    const M22 cov = add(scale(mul(T, Tt), r2), M22{pg.Cxx, pg.Cxy, pg.Cxy, pg.Cyy});
-    const M22 F = inverse(cov);
-    const float detF = det(F);
+    const M Tt = transpose(T);
+    const M cov = add(scale(mul(T, Tt), r2), M{pg.Cxx, pg.Cxy, pg.Cxy, pg.Cyy});
+    const M F = inverse(cov);
+    const double detF = det(F);
 
-    ImgGauss g;
-    g.xcen = xs + pg.xcen;
-    g.ycen = ys + pg.ycen;
-    g.fxx = F.a;
-    g.fyy = F.d;
-    g.fxy = F.c;
-    const float flux = src.fluxes[e.band];
-    const float am = src.mixture_amplitudes[bin];
-    const float tmp = e.G * pg.amp * sqrtf(detF) * (1.0f / (2.0f * 3.14159265358979323846f));
-    g.amp = tmp * flux * am;
-    g.hc = 1.0f - (F.a + F.d) * (1.0f / 24.0f);
-    g.pad0 = 0.0f;
+    const double flux = src.fluxes[e.band];
+    const double am = src.mixture_amplitudes[bin];
+    const double tmp = (double)e.G * (double)pg.amp * sqrt(detF) * (0.5 / 3.14159265358979323846);
+    const double amp = tmp * flux * am;
+    const double xc = xs + (double)pg.xcen;
+    const double yc = ys + (double)pg.ycen;
 
     // dSigma = r2 (T dT^T + dT T^T); dF = -F dSigma F
-    const M22 dSq = scale(add(mul(T, transpose(dT_dq)), mul(dT_dq, Tt)), r2);
-    const M22 dSp = scale(add(mul(T, transpose(dT_dpa)), mul(dT_dpa, Tt)), r2);
-    const M22 dFq = scale(mul(F, mul(dSq, F)), -1.0f);
-    const M22 dFp = scale(mul(F, mul(dSp, F)), -1.0f);
+    const M dSq = scale(add(mul(T, transpose(dT_dq)), mul(dT_dq, Tt)), r2);
+    const M dSp = scale(add(mul(T, transpose(dT_dpa)), mul(dT_dpa, Tt)), r2);
+    const M dFq = scale(mul(F, mul(dSq, F)), -1.0);
+    const M dFp = scale(mul(F, mul(dSp, F)), -1.0);
 
-    g.dA_dQ = 0.5f * g.amp * trace(mul(cov, dFq));
-    g.dA_dPA = 0.5f * g.amp * trace(mul(cov, dFp));
-    g.dA_dFlux = tmp * am;
-    g.dA_dSersic = tmp * flux * src.damplitude_dnsersic[bin];
-    g.dA_drh = tmp * flux * src.damplitude_drh[bin];
-    g.dFxx_dQ = dFq.a;
-    g.dFyy_dQ = dFq.d;
-    g.dFxy_dQ = dFq.c;
-    g.dFxx_dPA = dFp.a;
-    g.dFyy_dPA = dFp.d;
-    g.dFxy_dPA = dFp.c;
+    g.xcen = (float)xc;
+    g.ycen = (float)yc;
+    g.fxx = (float)F.a;
+    g.fyy = (float)F.d;
+    g.fxy = (float)F.c;
+    g.amp = (float)amp;
+    g.hc = (float)(1.0 - (F.a + F.d) * (1.0 / 24.0));
+    g.pad0 = 0.0f;
+    g.dA_dQ = (float)(0.5 * amp * trace(mul(cov, dFq)));
+    g.dA_dPA = (float)(0.5 * amp * trace(mul(cov, dFp)));
+    g.dA_dFlux = (float)(tmp * am);
+    g.dA_dSersic = (float)(tmp * flux * src.damplitude_dnsersic[bin]);
+    g.dA_drh = (float)(tmp * flux * src.damplitude_drh[bin]);
+    g.dFxx_dQ = (float)dFq.a;
+    g.dFyy_dQ = (float)dFq.d;
+    g.dFxy_dQ = (float)dFq.c;
+    g.dFxx_dPA = (float)dFp.a;
+    g.dFyy_dPA = (float)dFp.d;
+    g.dFxy_dPA = (float)dFp.c;
     g.pad1 = 0.0f;
-    g.dx_dAlpha = CW.a;
-    g.dy_dAlpha = CW.c;
-    g.dx_dDelta = CW.b;
-    g.dy_dDelta = CW.d;
-    return g;
+    g.dx_dAlpha = CWp[0];
+    g.dy_dAlpha = CWp[2];
+    g.dx_dDelta = CWp[1];
+    g.dy_dDelta = CWp[3];
+
+    g64.xcen = xc;
+    g64.ycen = yc;
+    g64.fxx = F.a;
+    g64.fyy = F.d;
+    g64.fxy = F.c;
+    g64.amp = amp;
+    g64.pad0 = 0.0;
+    g64.pad1 = 0.0;
 }
 
 // One workgroup per exposure; lanes stride over its n_src * n_psf Gaussians.
 __global__ __launch_bounds__(128) void convert_kernel(
     const ExpDesc *__restrict__ exps, const fpho_source *__restrict__ props,
-    ImgGauss *__restrict__ out) {
+    ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64) {
     const ExpDesc e = exps[blockIdx.x];
     const int n = e.n_src * e.n_psf;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const int s = j / e.n_psf;
         const int p = j - s * e.n_psf;
-        out[e.gauss_off + j] = make_image_gaussian(e, props[e.src_off + s], s, p);
+        ImgGauss g;
+        ImgGauss64 g64;
+        make_image_gaussian(e, props[e.src_off + s], s, p, g, g64);
+        out[e.gauss_off + j] = g;
+        out64[e.gauss_off + j] = g64;
     }
 }
 
@@ -132,23 +150,41 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Value of one image Gaussian at one pixel in FP64 (bright terms only).
+__device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, float yp) {
+    const double dx = (double)xp - g.xcen;
+    const double dy = (double)yp - g.ycen;
+    const double vx = g.fxx * dx + g.fxy * dy;
+    const double vy = g.fyy * dy + g.fxy * dx;
+    const double ea = dx * vx + dy * vy;
+    const double h = 1.0 + (vx * vx + vy * vy - g.fxx - g.fyy) * (1.0 / 24.0);
+    return g.amp * exp(-0.5 * ea) * h;
+}
+
 // Evaluate one wave tile.  `tile` must be wave-uniform (an SGPR value) so
 // every descriptor and Gaussian load below is a scalar load.
-template <int K>
+//   MIXED = true : residual accumulated in FP64; a term whose FP32 value
+//                  exceeds kBrightSigma * sigma of its pixel is re-evaluated
+//                  in FP64 (a wave branch taken only near bright cores).
+//   MIXED = false: FP32 residual, as forcepho/src/common_kernel.cc:57-83.
+template <int K, bool MIXED>
 __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__ tiles,
                                           const ExpDesc *__restrict__ exps,
                                           const ImgGauss *__restrict__ gauss,
+                                          const ImgGauss64 *__restrict__ gauss64,
                                           float *__restrict__ gpart,
                                           double *__restrict__ cpart) {
     const int lane = threadIdx.x & (kWave - 1);
     const TileDesc t = tiles[tile];
     const ExpDesc &e = exps[t.exp];
     const ImgGauss *__restrict__ g = gauss + e.gauss_off;
+    const ImgGauss64 *__restrict__ g64 = gauss64 + e.gauss_off;
     const int n_src = e.n_src;
     const int n_psf = e.n_psf;
     const int ng = n_src * n_psf;
 
-    float x[K], y[K], d[K], ie[K], r[K];
+    float x[K], y[K], d[K], ie[K], r[K], thr[K];
+    double rd[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = k * kWave + lane;
@@ -165,6 +201,8 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             ie[k] = 0.0f;
         }
         r[k] = d[k];
+        rd[k] = (double)d[k];
+        thr[k] = ie[k] > 0.0f ? kBrightSigma / ie[k] : __builtin_huge_valf();
     }
 
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
@@ -182,7 +220,15 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             if (!(ea > FPHO_MAX_EXP_ARG)) {
                 const float gp = __builtin_amdgcn_exp2f(ea * kHalfLog2e);
                 const float h = hc + (vx * vx + vy * vy) * (1.0f / 24.0f);
-                r[k] -= amp * gp * h;
+                const float c = amp * gp * h;
+                if (MIXED) {
+                    if (fabsf(c) > thr[k])
+                        rd[k] -= pair_value64(g64[i], x[k], y[k]);
+                    else
+                        rd[k] -= (double)c;
+                } else {
+                    r[k] -= c;
+                }
             }
         }
     }
@@ -193,13 +239,20 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = k * kWave + lane;
+        if (MIXED) {
+            r[k] = (float)rd[k];
+            const double rs = rd[k] * (double)ie[k];
+            const double ds = (double)d[k] * (double)ie[k];
+            c2 += (rs - ds) * (rs + ds);
+        } else {
+            const float rs = r[k] * ie[k];
+            const float ds = d[k] * ie[k];
+            double c = (double)rs - (double)ds;
+            c *= (double)rs + (double)ds;
+            c2 += c;
+        }
         if (e.residual != nullptr && p < t.npix) e.residual[t.pix0 + p] = r[k];
-        const float rs = r[k] * ie[k];
-        const float ds = d[k] * ie[k];
-        double c = (double)rs - (double)ds;
-        c *= (double)rs + (double)ds;
-        c2 += c;
-        rr[k] = -2.0f * (rs * ie[k]);  // -2 r ierr^2
+        rr[k] = -2.0f * ((r[k] * ie[k]) * ie[k]);  // -2 r ierr^2
     }
     c2 = wave_sum(c2);
     if (lane == 0) cpart[tile] = c2;
@@ -266,27 +319,27 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 }
 
 // Non-persistent: wave w of workgroup b evaluates queue[b*W + w].
-template <int K>
+template <int K, bool MIXED>
 __global__ __launch_bounds__(256) void eval_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t waves_per_group, const ExpDesc *__restrict__ exps,
-    const ImgGauss *__restrict__ gauss, float *__restrict__ gpart,
-    double *__restrict__ cpart) {
+    const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
+    float *__restrict__ gpart, double *__restrict__ cpart) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = blockIdx.x * waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-    eval_tile<K>(tile, tiles, exps, gauss, gpart, cpart);
+    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart);
 }
 
 // Persistent: every wave pulls tiles from `counter` until the queue is empty.
 // Each wave reaches the exit once the counter passes n_tiles.
-template <int K>
+template <int K, bool MIXED>
 __global__ __launch_bounds__(256) void eval_persistent_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t *__restrict__ counter, const ExpDesc *__restrict__ exps,
-    const ImgGauss *__restrict__ gauss, float *__restrict__ gpart,
-    double *__restrict__ cpart) {
+    const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
+    float *__restrict__ gpart, double *__restrict__ cpart) {
     const int lane = threadIdx.x & (kWave - 1);
     for (;;) {
         int slot = 0;
@@ -294,7 +347,7 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
         slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, kWave));
         if (slot >= n_tiles) break;
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-        eval_tile<K>(tile, tiles, exps, gauss, gpart, cpart);
+        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart);
     }
 }
 
@@ -491,39 +544,45 @@ __global__ __launch_bounds__(64) void stack_kernel(
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, hipStream_t stream) {
+                          ImgGauss *out, ImgGauss64 *out64, hipStream_t stream) {
     if (n_exp <= 0) return hipSuccess;
     hipLaunchKernelGGL(convert_kernel, dim3((unsigned)n_exp), dim3(128), 0, stream,
-                       exps, props, out);
+                       exps, props, out, out64);
     return hipGetLastError();
 }
 
-template <int K>
+template <int K, bool MIXED>
 static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
     const int W = a.waves_per_group;
     if (a.persistent) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
-        hipLaunchKernelGGL(eval_persistent_kernel<K>, dim3((unsigned)a.grid), dim3(64 * W), 0,
-                           stream, a.tiles, a.queue, (int32_t)a.n_tiles, a.counter, a.exps,
-                           a.gauss, a.gpart, a.cpart);
+        hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED>), dim3((unsigned)a.grid),
+                           dim3(64 * W), 0, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
+                           a.counter, a.exps, a.gauss, a.gauss64, a.gpart, a.cpart);
     } else {
         const int64_t groups = (a.n_tiles + W - 1) / W;
-        hipLaunchKernelGGL(eval_kernel<K>, dim3((unsigned)groups), dim3(64 * W), 0, stream,
-                           a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss,
-                           a.gpart, a.cpart);
+        hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), 0,
+                           stream, a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps,
+                           a.gauss, a.gauss64, a.gpart, a.cpart);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_eval(const EvalArgs &a, hipStream_t stream) {
-    if (a.n_tiles <= 0) return hipSuccess;
+template <bool MIXED>
+static hipError_t launch_eval_p(const EvalArgs &a, hipStream_t stream) {
     switch (a.pix_per_lane) {
-        case 1: return launch_eval_k<1>(a, stream);
-        case 2: return launch_eval_k<2>(a, stream);
-        case 4: return launch_eval_k<4>(a, stream);
-        case 8: return launch_eval_k<8>(a, stream);
+        case 1: return launch_eval_k<1, MIXED>(a, stream);
+        case 2: return launch_eval_k<2, MIXED>(a, stream);
+        case 4: return launch_eval_k<4, MIXED>(a, stream);
+        case 8: return launch_eval_k<8, MIXED>(a, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_eval(const EvalArgs &a, hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    return a.precision == kPrecMixed ? launch_eval_p<true>(a, stream)
+                                     : launch_eval_p<false>(a, stream);
 }
 
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,

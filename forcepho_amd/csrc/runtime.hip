@@ -129,6 +129,7 @@ struct Plan {
     DevVec<int32_t> queue;
     DevVec<OutDesc> outs;
     DevVec<ImgGauss> gauss;
+    DevVec<ImgGauss64> gauss64;
     DevVec<float> gpart;
     DevVec<double> cpart;
     DevVec<int32_t> counter;
@@ -138,8 +139,15 @@ struct Plan {
     int64_t n_pairs = 0, npix = 0;
     std::vector<int64_t> src_off, chi2_off, dchi2_off, theta_off;  // per patch, +1
     fpho_launch launch;
-    int reduce_fp64 = 1;
+    int precision = kPrecMixed;
     bool valid = false;
+    // optional per-stage timing: events before convert, eval, reduce and after
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Plan() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 struct fpho_patch {
@@ -156,6 +164,7 @@ struct fpho_patch {
     bool return_residual = false;
     uint64_t generation = 1;  // bumps on metadata change or data/residual swap
     fpho_launch launch = default_launch(false);
+    int precision = kPrecMixed;
     Plan plan;
     uint64_t plan_generation = 0;
     // staging for the synchronous drop-in
@@ -391,6 +400,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     HIP_TRY(plan.srcs.upload(hp.srcs.data(), (int64_t)hp.srcs.size()));
     HIP_TRY(plan.pouts.upload(hp.pouts.data(), (int64_t)hp.pouts.size()));
     HIP_TRY(plan.gauss.alloc(hp.n_gauss));
+    HIP_TRY(plan.gauss64.alloc(hp.n_gauss));
     HIP_TRY(plan.gpart.alloc(std::max<int64_t>(hp.n_part, 1)));
     HIP_TRY(plan.cpart.alloc(std::max<int64_t>((int64_t)hp.tiles.size(), 1)));
     HIP_TRY(plan.counter.alloc(1));
@@ -410,7 +420,9 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
 
 int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
              hipStream_t stream) {
-    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, stream));
+    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
+    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
+                           stream));
     EvalArgs a;
     a.tiles = plan.tiles.ptr;
     a.queue = plan.queue.ptr;
@@ -418,6 +430,8 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
     a.counter = plan.counter.ptr;
     a.exps = plan.exps.ptr;
     a.gauss = plan.gauss.ptr;
+    a.gauss64 = plan.gauss64.ptr;
+    a.precision = plan.precision;
     a.gpart = plan.gpart.ptr;
     a.cpart = plan.cpart.ptr;
     a.pix_per_lane = plan.launch.pix_per_lane;
@@ -427,9 +441,12 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
     a.grid = std::min<int64_t>(groups_needed,
                                (int64_t)plan.launch.groups_per_cu * cu_count_cached());
     if (a.grid < 1) a.grid = 1;
+    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[1], stream));
     HIP_TRY(launch_eval(a, stream));
+    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[2], stream));
     HIP_TRY(launch_reduce(plan.outs.ptr, plan.n_outs, plan.gpart.ptr, plan.cpart.ptr, chi2,
-                          dchi2, plan.reduce_fp64, stream));
+                          dchi2, plan.precision != kPrecFP32, stream));
+    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[3], stream));
     return FPHO_OK;
 }
 
@@ -438,6 +455,7 @@ int ensure_patch_plan(fpho_patch *p) {
     std::vector<fpho_patch *> one{p};
     const int rc = build_plan(one, p->launch, p->plan);
     if (rc) return rc;
+    p->plan.precision = p->precision;
     p->plan_generation = p->generation;
     return FPHO_OK;
 }
@@ -448,10 +466,10 @@ int ensure_batch_plan(fpho_batch *b) {
         stale = b->generations[i] != b->patches[i]->generation;
     if (!stale) return FPHO_OK;
     const fpho_launch l = b->plan.valid ? b->plan.launch : default_launch(true);
-    const int fp64 = b->plan.reduce_fp64;
+    const int prec = b->plan.precision;
     const int rc = build_plan(b->patches, l, b->plan);
     if (rc) return rc;
-    b->plan.reduce_fp64 = fp64;
+    b->plan.precision = prec;
     for (size_t i = 0; i < b->patches.size(); ++i) b->generations[i] = b->patches[i]->generation;
     return FPHO_OK;
 }
@@ -587,6 +605,14 @@ int fpho_patch_set_launch(fpho_patch *p, const fpho_launch *l) {
     return FPHO_OK;
 }
 
+int fpho_patch_set_precision(fpho_patch *p, int mode) {
+    if (!p) return fail(FPHO_E_ARG, "null patch");
+    if (mode < kPrecFP32 || mode > kPrecMixed) return fail(FPHO_E_ARG, "precision mode %d", mode);
+    p->precision = mode;
+    p->plan.precision = mode;
+    return FPHO_OK;
+}
+
 int fpho_patch_free(fpho_patch *p) {
     if (!p) return FPHO_OK;
     (void)hipDeviceSynchronize();
@@ -686,15 +712,16 @@ int fpho_batch_set_launch(fpho_batch *b, const fpho_launch *l) {
     if (!b || !l) return fail(FPHO_E_ARG, "null argument");
     const int rc = check_launch(*l);
     if (rc) return rc;
-    const int fp64 = b->plan.reduce_fp64;
+    const int prec = b->plan.precision;
     const int r2 = build_plan(b->patches, *l, b->plan);
-    b->plan.reduce_fp64 = fp64;
+    b->plan.precision = prec;
     return r2;
 }
 
-int fpho_batch_set_precision(fpho_batch *b, int reduce_fp64) {
+int fpho_batch_set_precision(fpho_batch *b, int mode) {
     if (!b) return fail(FPHO_E_ARG, "null batch");
-    b->plan.reduce_fp64 = reduce_fp64 ? 1 : 0;
+    if (mode < kPrecFP32 || mode > kPrecMixed) return fail(FPHO_E_ARG, "precision mode %d", mode);
+    b->plan.precision = mode;
     return FPHO_OK;
 }
 
@@ -704,6 +731,25 @@ int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *props, double *c
     const int rc = ensure_batch_plan(b);
     if (rc) return rc;
     return run_plan(b->plan, props, chi2, dchi2, (hipStream_t)stream);
+}
+
+int fpho_batch_set_timing(fpho_batch *b, int on) {
+    if (!b) return fail(FPHO_E_ARG, "null batch");
+    if (on) {
+        for (auto &e : b->plan.ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
+    }
+    b->plan.timing = on != 0;
+    return FPHO_OK;
+}
+
+int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
+    if (!b || !ms3) return fail(FPHO_E_ARG, "null argument");
+    if (!b->plan.timing) return fail(FPHO_E_STATE, "timing is not enabled");
+    HIP_TRY(hipEventSynchronize(b->plan.ev[3]));
+    for (int i = 0; i < 3; ++i)
+        HIP_TRY(hipEventElapsedTime(&ms3[i], b->plan.ev[i], b->plan.ev[i + 1]));
+    return FPHO_OK;
 }
 
 int fpho_batch_free(fpho_batch *b) {

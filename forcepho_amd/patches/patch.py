@@ -72,16 +72,13 @@ def superpixel_order(nx, ny, sx, sy):
     blocks taken row-major over the block grid, each block flattened
     row-major (forcepho/patches/storage.py:238-270).  Edge blocks are
     truncated when sx or sy does not divide the image."""
-    if nx % sx == 0 and ny % sy == 0:
-        idx = np.arange(nx * ny).reshape(nx // sx, sx, ny // sy, sy)
-        return idx.transpose(0, 2, 1, 3).reshape(-1)
-    order = []
-    for I in range(0, nx, sx):
-        for J in range(0, ny, sy):
-            ii, jj = np.meshgrid(np.arange(I, min(I + sx, nx)),
-                                 np.arange(J, min(J + sy, ny)), indexing="ij")
-            order.append((ii * ny + jj).reshape(-1))
-    return np.concatenate(order)
+    NX = -(-nx // sx) * sx
+    NY = -(-ny // sy) * sy
+    ii, jj = np.meshgrid(np.arange(NX), np.arange(NY), indexing="ij")
+    ii = ii.reshape(NX // sx, sx, NY // sy, sy).transpose(0, 2, 1, 3).reshape(-1)
+    jj = jj.reshape(NX // sx, sx, NY // sy, sy).transpose(0, 2, 1, 3).reshape(-1)
+    keep = (ii < nx) & (jj < ny)
+    return ii[keep] * ny + jj[keep]
 
 
 def pack_exposures(images, ierrs, bands, superpixel=(8, 8), warp_size=64,

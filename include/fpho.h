@@ -55,6 +55,18 @@ extern "C" {
 #define FPHO_E_STATE 4    /* object in the wrong state              */
 #define FPHO_E_VALUE 5    /* non-physical input value (q<=0, NaN)   */
 
+/* Precision modes.  Pair arithmetic (the bulk) is FP32 in all modes; the
+ * convert stage (per image Gaussian) is FP64 in all modes.
+ *   FPHO_PREC_FP32        FP32 residual, FP32 cross-tile sums (ablation)
+ *   FPHO_PREC_FP64_REDUCE FP32 residual as the reference kernel computes it
+ *                         (common_kernel.cc:57-83), FP64 cross-tile sums
+ *   FPHO_PREC_MIXED       FP64 residual accumulation; terms brighter than
+ *                         0.1 sigma of their pixel re-evaluated in FP64;
+ *                         FP64 cross-tile sums (default) */
+#define FPHO_PREC_FP32 0
+#define FPHO_PREC_FP64_REDUCE 1
+#define FPHO_PREC_MIXED 2
+
 /* == class Source, forcepho/src/proposal.cu:13-27 (264 bytes). */
 typedef struct fpho_source {
     float ra, dec;
@@ -139,6 +151,8 @@ int fpho_patch_read(fpho_patch *p, int which, float *host_out);
 int fpho_patch_info(const fpho_patch *p, int32_t *n_bands, int32_t *n_sources,
                     int64_t *npix, int64_t *n_pairs);
 int fpho_patch_set_launch(fpho_patch *p, const fpho_launch *launch);
+/* Precision of the pixel pass and reductions (FPHO_PREC_*). */
+int fpho_patch_set_precision(fpho_patch *p, int mode);
 int fpho_patch_free(fpho_patch *p);
 
 /* Synchronous drop-in: proposal (host, n_sources entries) in, chi2[n_bands]
@@ -168,10 +182,15 @@ int fpho_batch_layout(const fpho_batch *b, int64_t *src_off, int64_t *chi2_off,
 int fpho_batch_info(const fpho_batch *b, int64_t *n_tiles, int64_t *n_pairs,
                     int64_t *npix, int64_t *n_gauss);
 int fpho_batch_set_launch(fpho_batch *b, const fpho_launch *launch);
-/* reduce_fp64 = 1 (default): tile partials summed in FP64; 0: all FP32. */
-int fpho_batch_set_precision(fpho_batch *b, int reduce_fp64);
+int fpho_batch_set_precision(fpho_batch *b, int mode);
 int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *proposals_dev,
                               double *chi2_dev, double *dchi2_dev, void *stream);
+/* Record HIP events around the three stages of every evaluation (on the
+ * evaluation's stream); fpho_batch_last_timing waits for the last evaluation
+ * and returns milliseconds of convert, pixel pass and reduce.  Timing adds
+ * event records to the stream; leave it off when capturing graphs. */
+int fpho_batch_set_timing(fpho_batch *b, int on);
+int fpho_batch_last_timing(fpho_batch *b, float *ms3);
 int fpho_batch_free(fpho_batch *b);
 
 /* ------------------------------------------ device proposals and gradients */

@@ -246,7 +246,9 @@ def pixel_values_np(par, xp, yp, cutoff=True):
 
 def evaluate_np(patch, proposal, cutoff=True):
     """(chi2 [n_bands], dchi2 [n_bands, n_sources, 7], residual [npix]) in
-    vectorised FP64 numpy; suitable for small patches."""
+    vectorised FP64 numpy; suitable for small patches.  ``proposal`` is a
+    Source-struct array or a list of dicts with the same field names (FP64
+    values)."""
     a = _arrays(patch)
     ns = int(patch.n_sources)
     nb = len(a["band_N"])
@@ -343,3 +345,74 @@ class ReferenceKernel:
 
 def reference_evaluate(patch, proposal):
     return ReferenceKernel(patch).evaluate(proposal)
+
+
+# --------------------------------------------- parameter vector, all FP64
+
+_SPLINES = None
+
+
+def scipy_splines():
+    """The reference's amplitude interpolants: one scipy RectBivariateSpline
+    per mixture component over the table (forcepho/sources.py:623-645)."""
+    global _SPLINES
+    if _SPLINES is None:
+        from scipy.interpolate import RectBivariateSpline
+        table = os.path.join(os.path.dirname(HERE), "forcepho_amd", "data",
+                             "sersic_splinedata_large.npz")
+        with np.load(table) as f:
+            n, r, A, radii = f["nsersic"], f["rh"], f["amplitudes"], f["radii"]
+        ng, rg = np.unique(n), np.unique(r)
+        sp = [RectBivariateSpline(ng, rg, A[:, i].reshape(len(ng), len(rg)), s=None)
+              for i in range(A.shape[1])]
+        _SPLINES = (sp, float(n.min()), np.asarray(radii, dtype=np.float64))
+    return _SPLINES
+
+
+def amplitudes_scipy(nsersic, rh):
+    """(a, da/dnsersic, da/drh) with the branches of forcepho/sources.py:656-694."""
+    sp, nmin, radii = scipy_splines()
+    z = np.zeros(len(sp))
+    if nsersic < nmin:
+        a = z.copy()
+        a[np.argmin(np.abs(rh - radii))] = 1.0
+        return a, z.copy(), z.copy()
+    if rh < 0:
+        a = z.copy()
+        a[0] = 1.0
+        return a, z.copy(), z.copy()
+    a = np.array([s(nsersic, rh)[0, 0] for s in sp])
+    dn = np.array([s(nsersic, rh, dx=1)[0, 0] for s in sp])
+    dr = np.array([s(nsersic, rh, dy=1)[0, 0] for s in sp])
+    return a, dn, dr
+
+
+def sources_from_theta(theta, n_bands):
+    """FP64 source dicts from [flux_1..flux_NB, ra, dec, q, pa, n, rh] blocks."""
+    theta = np.asarray(theta, dtype=np.float64).reshape(-1, n_bands + 6)
+    out = []
+    for t in theta:
+        a, dn, dr = amplitudes_scipy(t[n_bands + 4], t[n_bands + 5])
+        out.append({"fluxes": t[:n_bands], "ra": t[n_bands], "dec": t[n_bands + 1],
+                    "q": t[n_bands + 2], "pa": t[n_bands + 3],
+                    "mixture_amplitudes": a, "damplitude_dnsersic": dn,
+                    "damplitude_drh": dr})
+    return out
+
+
+def stack_grad(dchi2):
+    """[n_bands, n_sources, 7] -> per source [flux_b..., ra, dec, q, pa, n, rh]
+    (forcepho/model.py:370-387)."""
+    nb, ns, _ = dchi2.shape
+    g = np.zeros((ns, nb + 6))
+    for b in range(nb):
+        g[:, b] += dchi2[b, :, 0]
+        g[:, nb:] += dchi2[b, :, 1:]
+    return g.reshape(-1)
+
+
+def lnlike_theta_np(patch, theta, cutoff=True):
+    """lnlike = -chi2/2 (summed over bands) and d lnlike / d theta in FP64."""
+    nb = len(np.asarray(patch.band_N))
+    chi2, dchi2, _ = evaluate_np(patch, sources_from_theta(theta, nb), cutoff=cutoff)
+    return -0.5 * chi2.sum(), -0.5 * stack_grad(dchi2)

@@ -39,3 +39,24 @@ def grad_mismatch(d, d_ref):
 
 def chi2_relerr(c, c_ref):
     return np.abs(np.asarray(c) - c_ref) / np.abs(c_ref)
+
+
+def small_uneven_specs():
+    """Four small uneven patches (configuration-4 shape, sizes the oracle
+    evaluates in well under a second)."""
+    from forcepho_amd.synth import PatchSpec
+    return [
+        PatchSpec(n_sources=3, n_bands=2, n_exp_per_band=[2, 1], side=48, n_psf=[3, 4]),
+        PatchSpec(n_sources=1, n_bands=1, n_exp_per_band=1, side=32, n_psf=3),
+        PatchSpec(n_sources=6, n_bands=3, n_exp_per_band=[1, 3, 2], side=64, n_psf=[5, 3, 6],
+                  psf_per_exposure=True, psf_elliptical=True, dither=1.5),
+        PatchSpec(n_sources=2, n_bands=1, n_exp_per_band=4, side=40, n_psf=4, mask_frac=0.05),
+    ]
+
+
+def make_small_patches(config=40):
+    from forcepho_amd import synth
+    out = []
+    for i, spec in enumerate(small_uneven_specs()):
+        out.append(synth.make_patch(spec, config=config, patch_id=i, render=oracle_render))
+    return out

@@ -1,0 +1,132 @@
+"""The C-ABI library: it loads, exports every entry point include/fpho.h
+declares, its struct layouts match the ctypes/numpy mirrors, and input
+validation fails with error codes before any device work (CPU only)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from forcepho_amd import _lib, limits
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fpho.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(fpho_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTED)
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "fpho.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(fpho_source), sizeof(fpho_psf_gauss),
+         sizeof(fpho_patch_desc), sizeof(fpho_launch));
+  printf("%zu %zu %zu %zu\n", offsetof(fpho_patch_desc, n_bands),
+         offsetof(fpho_patch_desc, n_sources), offsetof(fpho_patch_desc, psfgauss),
+         offsetof(fpho_patch_desc, psfgauss_start));
+  printf("%zu %zu\n", offsetof(fpho_source, fluxes), offsetof(fpho_source, damplitude_dnsersic));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    v = list(map(int, out))
+    assert v[0] == limits.source_struct_dtype.itemsize == 264
+    assert v[1] == limits.psf_struct_dtype.itemsize == 28
+    assert v[2] == ctypes.sizeof(_lib.PatchDesc)
+    assert v[3] == ctypes.sizeof(_lib.Launch)
+    assert v[4] == _lib.PatchDesc.n_bands.offset
+    assert v[5] == _lib.PatchDesc.n_sources.offset
+    assert v[6] == _lib.PatchDesc.psfgauss.offset
+    assert v[7] == _lib.PatchDesc.psfgauss_start.offset
+    assert v[8] == limits.source_struct_dtype.fields["fluxes"][1]
+    assert v[9] == limits.source_struct_dtype.fields["damplitude_dnsersic"][1]
+
+
+def _tiny_desc(keep, **over):
+    npix = 64
+    arrs = dict(data=np.zeros(npix, np.float32), ierr=np.ones(npix, np.float32),
+                xpix=np.zeros(npix, np.float32), ypix=np.zeros(npix, np.float32),
+                band_start=np.zeros(1, np.int16), band_N=np.ones(1, np.int16),
+                exposure_start=np.zeros(1, np.int32), exposure_N=np.full(1, npix, np.int32),
+                rad2=np.ones(1, np.float32), D=np.eye(2, dtype=np.float32).ravel(),
+                crpix=np.zeros(2, np.float32), crval=np.zeros(2, np.float32),
+                CW=np.eye(2, dtype=np.float32).ravel(), G=np.ones(1, np.float32),
+                n_psf_per_source=np.ones(1, np.int32), psfgauss_start=np.zeros(1, np.int32))
+    psf = np.zeros(1, dtype=limits.psf_struct_dtype)
+    psf["amp"], psf["Cxx"], psf["Cyy"] = 1.0, 1.0, 1.0
+    arrs["psfgauss"] = psf
+    arrs.update(over)
+    keep.update(arrs)
+    d = _lib.PatchDesc()
+    d.npix = len(arrs["data"])
+    for k in ("data", "ierr", "xpix", "ypix", "rad2", "D", "crpix", "crval", "CW", "G"):
+        setattr(d, k, _lib.fptr(arrs[k], ctypes.c_float))
+    for k in ("band_start", "band_N"):
+        setattr(d, k, _lib.fptr(arrs[k], ctypes.c_int16))
+    for k in ("exposure_start", "exposure_N", "n_psf_per_source", "psfgauss_start"):
+        setattr(d, k, _lib.fptr(arrs[k], ctypes.c_int32))
+    d.n_bands, d.n_exp, d.n_sources, d.n_radii = 1, 1, 1, 1
+    d.psfgauss = ctypes.c_void_p(arrs["psfgauss"].ctypes.data)
+    d.n_psfgauss = len(arrs["psfgauss"])
+    return d
+
+
+@pytest.mark.parametrize("case,code", [
+    ("nan_pixel", 5), ("negative_ierr", 5), ("exposure_overrun", 1), ("psf_range", 1),
+    ("psf_not_pd", 5), ("too_many_sources", 2), ("bad_bin", 1)])
+def test_validation_errors_before_device_work(case, code):
+    keep = {}
+    if case == "nan_pixel":
+        data = np.zeros(64, np.float32)
+        data[5] = np.nan
+        d = _tiny_desc(keep, data=data)
+    elif case == "negative_ierr":
+        ie = np.ones(64, np.float32)
+        ie[3] = -1
+        d = _tiny_desc(keep, ierr=ie)
+    elif case == "exposure_overrun":
+        d = _tiny_desc(keep, exposure_N=np.full(1, 65, np.int32))
+    elif case == "psf_range":
+        d = _tiny_desc(keep, psfgauss_start=np.full(1, 1, np.int32))
+    elif case == "psf_not_pd":
+        psf = np.zeros(1, dtype=limits.psf_struct_dtype)
+        psf["Cxx"], psf["Cyy"], psf["Cxy"] = 1.0, 1.0, 2.0
+        d = _tiny_desc(keep, psfgauss=psf)
+    elif case == "too_many_sources":
+        d = _tiny_desc(keep)
+        d.n_sources = limits.MAX_NATIVE_SOURCES + 1
+    elif case == "bad_bin":
+        psf = np.zeros(1, dtype=limits.psf_struct_dtype)
+        psf["Cxx"], psf["Cyy"], psf["sersic_bin"] = 1.0, 1.0, 3
+        d = _tiny_desc(keep, psfgauss=psf)
+    h = ctypes.c_void_p()
+    rc = _lib.lib().fpho_patch_create(ctypes.byref(d), 0, ctypes.byref(h))
+    assert rc == code, _lib.lib().fpho_last_error()
+    assert not h.value
+    assert len(_lib.lib().fpho_last_error()) > 0
+
+
+def test_launch_validation():
+    l = _lib.Launch(3, 4, 0, 8)
+    rc = _lib.lib().fpho_patch_set_launch(ctypes.c_void_p(0), ctypes.byref(l))
+    assert rc == 1  # null patch is rejected first

@@ -1,0 +1,94 @@
+"""torch.autograd adjoint on the device: proposals built on the device from
+the parameter vector (FP64 splines), lnlike = -chi2/2 and its gradient
+against the FP64 oracle, and backward scaling."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from forcepho_amd import synth
+from forcepho_amd.autograd import LnLike
+from forcepho_amd.batch import DeviceSplines, PatchBatch
+from forcepho_amd.limits import source_struct_dtype
+from helpers import CHI2_RTOL, make_config_patch, proposal_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1_batch(hip):
+    patch, scene, theta = make_config_patch(1)
+    patch.send_to_device()
+    batch = PatchBatch([patch])
+    splines = DeviceSplines()
+    yield patch, scene, theta, batch, splines
+    batch.free()
+    patch.free()
+
+
+def _stacked_tolerance_ok(g, g_ref):
+    err = np.abs(g - g_ref)
+    return np.all((err <= 1e-4 * np.abs(g_ref)) | (err <= 1e-6 * np.abs(g_ref).max()))
+
+
+def test_device_proposals_match_host(c1_batch):
+    patch, scene, theta, batch, splines = c1_batch
+    thetas = synth.make_proposals(theta, 1, 5, config=21)
+    for t in thetas:
+        tt = torch.as_tensor(t, dtype=torch.float64, device="cuda")
+        batch.theta_to_proposals(tt, splines)
+        torch.cuda.synchronize()
+        dev = batch.proposals.cpu().numpy().view(source_struct_dtype)
+        host = proposal_for(scene, t)
+        for f in ("ra", "dec", "q", "pa", "nsersic", "rh", "fluxes"):
+            assert np.array_equal(dev[f], host[f]), f
+        for f in ("mixture_amplitudes", "damplitude_drh", "damplitude_dnsersic"):
+            a, b = dev[f].astype(np.float64), host[f].astype(np.float64)
+            assert np.all(np.abs(a - b) <= 2e-7 * np.maximum(np.abs(b), 1e-3)), f
+
+
+def test_lnlike_and_adjoint_vs_fp64_oracle_50_proposals(c1_batch):
+    """Configuration 3 on configuration 1's patch (14 parameters), 50 seeded
+    proposals: lnlike and gradient against the all-FP64 oracle (scipy splines,
+    FP64 source parameters)."""
+    patch, scene, theta, batch, splines = c1_batch
+    thetas = synth.make_proposals(theta, 1, 50, config=3)
+    for k, t in enumerate(thetas):
+        tt = torch.as_tensor(t, dtype=torch.float64, device="cuda").requires_grad_(True)
+        ll = LnLike.apply(tt, batch, splines)
+        ll.sum().backward()
+        ll_ref, g_ref = oracle.lnlike_theta_np(patch, t)
+        assert abs(ll.item() - ll_ref) <= CHI2_RTOL * abs(ll_ref), (k, ll.item(), ll_ref)
+        assert _stacked_tolerance_ok(tt.grad.cpu().numpy(), g_ref), k
+
+
+def test_backward_scales_incoming_gradient(c1_batch):
+    patch, scene, theta, batch, splines = c1_batch
+    tt = torch.as_tensor(theta, dtype=torch.float64, device="cuda").requires_grad_(True)
+    (3.0 * LnLike.apply(tt, batch, splines).sum()).backward()
+    g3 = tt.grad.clone()
+    tt.grad = None
+    LnLike.apply(tt, batch, splines).sum().backward()
+    assert torch.equal(g3, 3.0 * tt.grad)
+
+
+def test_config2_280_parameters(hip):
+    """20 sources x (8 fluxes + 6 shape) = 280 parameters on configuration 2's
+    patch, against the FP64 C oracle fed the same float32 proposal."""
+    patch, scene, theta = make_config_patch(2)
+    patch.send_to_device()
+    batch = PatchBatch([patch])
+    splines = DeviceSplines()
+    assert batch.n_theta == 280
+    for t in synth.make_proposals(theta, 8, 2, config=3, patch_id=1):
+        tt = torch.as_tensor(t, dtype=torch.float64, device="cuda").requires_grad_(True)
+        ll = LnLike.apply(tt, batch, splines)
+        ll.sum().backward()
+        prop = batch.proposals.cpu().numpy().view(source_struct_dtype)
+        c, d = oracle.evaluate(patch, prop)
+        ll_ref = -0.5 * c.sum()
+        g_ref = -0.5 * oracle.stack_grad(d)
+        assert abs(ll.item() - ll_ref) <= CHI2_RTOL * abs(ll_ref)
+        assert _stacked_tolerance_ok(tt.grad.cpu().numpy(), g_ref)
+    batch.free()
+    patch.free()

@@ -1,0 +1,189 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is chosen so that a plausible mistake in the restatement fails
+at least one of them: the compiled reference kernel's own outputs (golden
+fixture and live), an independent numpy formulation, central finite
+differences of every gradient component, and closed forms (normalisation,
+rotation invariance at q = 1, permutation and padding invariance)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from forcepho_amd import synth
+from helpers import make_config_patch, proposal_for
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return make_config_patch(1)
+
+
+@pytest.fixture(scope="module")
+def small_multiband():
+    spec = synth.PatchSpec(n_sources=3, n_bands=2, n_exp_per_band=[2, 1], side=40,
+                           n_psf=[2, 3], superpixel=(8, 8), warp_size=64, mask_frac=0.05,
+                           psf_per_exposure=True, psf_elliptical=True, dither=1.0)
+    from helpers import oracle_render
+    return synth.make_patch(spec, config=7, patch_id=0, render=oracle_render)
+
+
+def test_c_matches_numpy_restatement(c1, small_multiband):
+    for patch, scene, theta in (c1, small_multiband):
+        prop = proposal_for(scene, synth.make_proposals(theta, len(patch.band_N), 1,
+                                                        config=9)[0])
+        c, d, r = oracle.evaluate(patch, prop, residual=True)
+        cn, dn, rn = oracle.evaluate_np(patch, prop)
+        assert np.allclose(c, cn, rtol=1e-12, atol=0)
+        assert np.abs(d - dn).max() <= 1e-10 * np.abs(dn).max()
+        assert np.abs(r - rn).max() <= 1e-12 * np.abs(rn).max()
+
+
+def test_against_reference_golden(c1):
+    """Outputs of forcepho's own compiled CPU kernel (tests/golden/
+    reference_c1.npz) agree with the FP64 oracle to FP32 accuracy."""
+    patch, scene, theta = c1
+    g = np.load(os.path.join(GOLDEN, "reference_c1.npz"))
+    assert np.isclose(float(g["data_checksum"]), np.sum(patch.data, dtype=np.float64),
+                      rtol=1e-12), "synthetic patch changed"
+    for t, c_ref, d_ref in zip(g["theta"], g["chi2"], g["dchi2"]):
+        c, d = oracle.evaluate(patch, proposal_for(scene, t))
+        assert np.all(np.abs(c - c_ref) <= 1e-6 * np.abs(c))
+        assert np.abs(d - d_ref).max() <= 2e-5 * np.abs(d).max()
+
+
+def test_against_reference_kernel_live(small_multiband):
+    if oracle.reference_module() is None:
+        pytest.skip("compiled reference kernel not built")
+    patch, scene, theta = small_multiband
+    prop = proposal_for(scene, theta)
+    c, d = oracle.evaluate(patch, prop)
+    cr, dr = oracle.ReferenceKernel(patch).evaluate(prop)
+    assert np.all(np.abs(c - cr) <= 1e-6 * np.abs(c))
+    for b in range(len(c)):
+        assert np.abs(d[b] - dr[b]).max() <= 2e-5 * np.abs(d[b]).max()
+
+
+@pytest.mark.parametrize("cutoff,rel_step,tol", [(False, 1e-5, 1e-6), (True, 1e-6, 1e-6)])
+def test_gradient_central_differences(c1, cutoff, rel_step, tol):
+    """d lnlike / d theta (FP64, through the amplitude splines) against central
+    differences.  With the 6-sigma cutoff the likelihood has tiny steps where
+    pairs cross the cutoff, so that case uses a smaller step."""
+    patch, scene, theta = c1
+    t0 = synth.make_proposals(theta, 1, 1, config=11)[0]
+    ll, g = oracle.lnlike_theta_np(patch, t0, cutoff=cutoff)
+    for i in range(len(t0)):
+        h = rel_step * abs(t0[i])
+        tp, tm = t0.copy(), t0.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (oracle.lnlike_theta_np(patch, tp, cutoff)[0] -
+              oracle.lnlike_theta_np(patch, tm, cutoff)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= tol * max(abs(g[i]), 1e-3 * np.abs(g).max()), (i, fd, g[i])
+
+
+def test_planted_mistake_is_caught(c1, monkeypatch):
+    """Dropping the second-order term from dC/dx in the numpy oracle must be
+    detected by the finite-difference check."""
+    patch, scene, theta = c1
+    orig = oracle.pixel_values_np
+
+    def wrong(par, xp, yp, cutoff=True):
+        C, grad = orig(par, xp, yp, cutoff)
+        amp, xc, yc, fxx, fyy, fxy = par
+        dx, dy = xp - xc, yp - yc
+        vx = fxx * dx + fxy * dy
+        vy = fyy * dy + fxy * dx
+        Gp = np.exp(-0.5 * (dx * vx + dy * vy))
+        grad = grad.copy()
+        grad[1] += amp * Gp * (fxx * vx + fxy * vy) / 12.  # undo the H correction
+        return C, grad
+
+    monkeypatch.setattr(oracle, "pixel_values_np", wrong)
+    t0 = theta.copy()
+    i = 1  # ra
+    ll, g = oracle.lnlike_theta_np(patch, t0, cutoff=False)
+    h = 1e-5 * abs(t0[i])
+    tp, tm = t0.copy(), t0.copy()
+    tp[i] += h
+    tm[i] -= h
+    fd = (oracle.lnlike_theta_np(patch, tp, False)[0] -
+          oracle.lnlike_theta_np(patch, tm, False)[0]) / (2 * h)
+    assert abs(fd - g[i]) > 1e-6 * abs(g[i])
+
+
+def _single_source_patch(q=0.7, pa=0.3, rh=0.09, psf_sigma=2.0, side=96):
+    spec = synth.PatchSpec(n_sources=1, n_bands=1, n_exp_per_band=1, side=side, n_psf=2,
+                           superpixel=(8, 8), warp_size=64, psf_sigma=(psf_sigma, psf_sigma))
+    patch, scene, theta = synth.make_patch(spec, config=13, patch_id=0)
+    # move the source to the image centre: sky offset = CW^-1 (centre - crpix)
+    theta = theta.copy()
+    cw = patch.CW[0, 0].astype(np.float64)
+    shift = np.array([(side - 1) / 2.0, (side - 1) / 2.0]) - patch.crpix[0, 0]
+    theta[1:3] = patch.crval[0, 0] + np.linalg.solve(cw, shift)
+    theta[3], theta[4] = q, pa
+    theta[5], theta[6] = 0.5, rh   # nsersic below the table: one-hot amplitudes
+    return patch, scene, theta
+
+
+def test_model_normalisation():
+    """The second-order pixel-integral term integrates to one, so a resolved,
+    contained source sums to flux * G * sum(psf amplitudes) = flux."""
+    patch, scene, theta = _single_source_patch()
+    prop = proposal_for(scene, theta)
+    _, _, m = oracle.evaluate(patch, prop, cutoff=False, model=True)
+    # the one-hot component's PSF block (float32 values, as packed)
+    psf = patch.psfgauss[patch.psfgauss["sersic_bin"] == 4]
+    flux = float(prop["fluxes"][0][0]) * np.sum(psf["amp"].astype(np.float64))
+    assert abs(m.sum() - flux) <= 1e-10 * flux
+
+
+def test_rotation_invariance_at_unit_axis_ratio():
+    """q = 1 with isotropic D and PSF: the model does not depend on pa, so
+    dchi2/dpa vanishes (a sign or transposition error in dT/dpa breaks it),
+    while at q = 0.7 it does not."""
+    patch, scene, theta = _single_source_patch(q=0.7, pa=0.0)
+    patch.D[...] = np.eye(2) / synth.SCALE
+    patch.CW[...] = 3600.0 * np.eye(2) / synth.SCALE * np.diag(
+        [np.cos(np.deg2rad(synth.SKY_ZERO[1])), 1.0])
+    # data: the elliptical source at pa = 0
+    _, _, m = oracle.evaluate(patch, proposal_for(scene, theta), cutoff=False, model=True)
+    patch.data = m.astype(np.float32)
+    t1 = theta.copy()
+    t1[3], t1[4] = 1.0, 0.4
+    c, d = oracle.evaluate(patch, proposal_for(scene, t1), cutoff=False)
+    assert abs(d[0, 0, 4]) <= 1e-9 * np.abs(d).max()
+    t2 = theta.copy()
+    t2[4] = 0.4
+    c, d = oracle.evaluate(patch, proposal_for(scene, t2), cutoff=False)
+    assert abs(d[0, 0, 4]) > 1e-3 * np.abs(d).max()
+
+
+def test_permutation_and_padding_invariance(c1):
+    patch, scene, theta = c1
+    prop = proposal_for(scene, theta)
+    c0, d0 = oracle.evaluate(patch, prop)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(patch.npix)
+    saved = {k: getattr(patch, k).copy() for k in ("data", "ierr", "xpix", "ypix")}
+    try:
+        for k in saved:
+            setattr(patch, k, saved[k][perm])
+        c1_, d1 = oracle.evaluate(patch, prop)
+        assert np.allclose(c1_, c0, rtol=1e-12)
+        assert np.abs(d1 - d0).max() <= 1e-10 * np.abs(d0).max()
+        # 64 extra padding pixels (ierr = 0) change nothing
+        for k in saved:
+            pad = np.full(64, -1.0 if k in ("xpix", "ypix") else 0.0, dtype=np.float32)
+            setattr(patch, k, np.concatenate([saved[k], pad]))
+        patch.exposure_N = patch.exposure_N + 64
+        c2_, d2 = oracle.evaluate(patch, prop)
+        assert np.allclose(c2_, c0, rtol=1e-13)
+        assert np.abs(d2 - d0).max() <= 1e-12 * np.abs(d0).max()
+    finally:
+        for k, v in saved.items():
+            setattr(patch, k, v)
+        patch.exposure_N = np.array([len(saved["data"])], dtype=np.int32)
