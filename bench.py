@@ -47,7 +47,25 @@ def parse():
     ap.add_argument("--launch", type=str, default=None,
                     help="K,W,persistent,groups_per_cu (default: autotuner cache)")
     ap.add_argument("--precision", default="mixed", choices=["fp32", "fp64_reduce", "mixed"])
+    ap.add_argument("--tune", type=str, default=None, metavar="CACHE_JSON",
+                    help="run the launch autotuner on this workload, write the "
+                         "cache to CACHE_JSON and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates the HBM "
+                         "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of 1 GiB, five times: every dispatch reads
+    and writes exactly 2^30 bytes."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 5}), flush=True)
 
 
 def algorithmic_counts(specs):
@@ -137,6 +155,9 @@ def cpu_baseline(patches, specs, thetas, seconds):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
                           "part of it (patches would be sharded across GPUs, see "
@@ -169,6 +190,17 @@ def main():
     thetas = [synth.make_proposals(th, len(p.band_N), 1, config=4, patch_id=i)[0]
               for i, (p, sc, th) in enumerate(patches)]
     theta = torch.as_tensor(np.concatenate(thetas), dtype=torch.float64, device="cuda")
+
+    if args.tune:
+        from forcepho_amd import autotune
+        autotune.CACHE = args.tune
+        best, table = autotune.tune(batch, lambda: batch.lnlike_and_grad(theta, splines),
+                                    reps=3, log=log)
+        autotune.save("batch", best, table[0][0])
+        log(f"best launch {best}: {table[0][0]:.3f} ms")
+        print(json.dumps({"best": best, "ms": table[0][0],
+                          "table": [[ms, c] for ms, c in table]}))
+        return
 
     # ---- batched: warm-up, then exactly `steps` timed sweeps
     for _ in range(args.warmup):

@@ -66,7 +66,7 @@ class PatchDesc(ctypes.Structure):
 class Launch(ctypes.Structure):
     """== fpho_launch (include/fpho.h)."""
     _fields_ = [("pix_per_lane", c_i32), ("waves_per_group", c_i32),
-                ("persistent", c_i32), ("groups_per_cu", c_i32)]
+                ("persistent", c_i32), ("groups_per_cu", c_i32), ("cull", c_i32)]
 
 
 _LIB = None

@@ -15,8 +15,10 @@ import json
 import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
-DEFAULT = {"batch": dict(pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8),
-           "patch": dict(pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8)}
+DEFAULT = {"batch": dict(pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8,
+                         cull=1),
+           "patch": dict(pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8,
+                         cull=1)}
 
 __all__ = ["candidates", "tune", "load_or_default", "save"]
 
@@ -27,8 +29,9 @@ def candidates(kind="batch"):
         if kind == "batch":
             for G in (4, 8, 16):
                 out.append(dict(pix_per_lane=K, waves_per_group=W, persistent=1,
-                                groups_per_cu=G))
-        out.append(dict(pix_per_lane=K, waves_per_group=W, persistent=0, groups_per_cu=8))
+                                groups_per_cu=G, cull=1))
+        out.append(dict(pix_per_lane=K, waves_per_group=W, persistent=0, groups_per_cu=8,
+                        cull=1))
     return out
 
 

@@ -105,11 +105,12 @@ class PatchBatch:
         self.precision = mode
         _lib.check(_lib.lib().fpho_batch_set_precision(self.handle, _lib.PRECISION[mode]))
 
-    def set_launch(self, pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8):
-        l = _lib.Launch(pix_per_lane, waves_per_group, persistent, groups_per_cu)
+    def set_launch(self, pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8,
+                   cull=1):
+        l = _lib.Launch(pix_per_lane, waves_per_group, persistent, groups_per_cu, cull)
         _lib.check(_lib.lib().fpho_batch_set_launch(self.handle, ctypes.byref(l)))
         self.launch = dict(pix_per_lane=pix_per_lane, waves_per_group=waves_per_group,
-                           persistent=persistent, groups_per_cu=groups_per_cu)
+                           persistent=persistent, groups_per_cu=groups_per_cu, cull=cull)
 
     def set_timing(self, on=True):
         _lib.check(_lib.lib().fpho_batch_set_timing(self.handle, int(bool(on))))

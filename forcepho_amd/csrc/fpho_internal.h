@@ -19,8 +19,10 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define FPHO_HD __host__ __device__
+#define FPHO_HDI __host__ __device__ inline
 #else
 #define FPHO_HD
+#define FPHO_HDI inline
 #endif
 
 namespace fpho {
@@ -31,19 +33,31 @@ constexpr int kNParams = FPHO_NPARAMS;
 // One image-plane Gaussian (a source Sersic component convolved with one PSF
 // component) plus the Jacobian of its 6 image parameters with respect to the
 // 7 on-sky source parameters.  Same quantities as ImageGaussian in
-// forcepho/src/common_kernel.cc:20-46, re-laid-out hot/cold for scalar loads;
-// hc = 1 - (fxx + fyy)/24 folds the constant part of the pixel-integral
-// correction H.
+// forcepho/src/common_kernel.cc:20-46, re-laid-out for scalar loads:
+//   box   : bounding box of the cutoff ellipse (exp argument <= 36, widened by
+//           a safety margin), as order-preserving integer keys of the float
+//           coordinates, so the per-tile skip test runs on the scalar ALU;
+//   hot   : what the residual pass needs; hc = 1 - (fxx + fyy)/24 folds the
+//           constant part of the pixel-integral correction H;
+//   cold  : the Jacobian, read by the gradient pass.
 struct alignas(16) ImgGauss {
-    // hot: residual pass (32 bytes)
-    float xcen, ycen, fxx, fyy, fxy, amp, hc, pad0;
-    // cold: gradient pass (64 bytes)
-    float dA_dFlux, dA_dQ, dFxx_dQ, dFyy_dQ;
+    int32_t xlo, xhi, ylo, yhi;                 // box (16 bytes)
+    float xcen, ycen, fxx, fyy, fxy, amp, hc, pad0;   // hot (32 bytes)
+    float dA_dFlux, dA_dQ, dFxx_dQ, dFyy_dQ;    // cold (64 bytes)
     float dFxy_dQ, dA_dPA, dFxx_dPA, dFyy_dPA;
     float dFxy_dPA, dA_dSersic, dA_drh, pad1;
     float dx_dAlpha, dy_dAlpha, dx_dDelta, dy_dDelta;
+    float pad2[4];
 };
-static_assert(sizeof(ImgGauss) == 96, "ImgGauss must be 96 bytes");
+static_assert(sizeof(ImgGauss) == 128, "ImgGauss must be 128 bytes");
+
+// Order-preserving map of a float to a signed integer (a < b <=> key(a) <
+// key(b) for all non-NaN floats), so ranges compare with integer instructions.
+FPHO_HDI int32_t order_key(float f) {
+    int32_t i;
+    __builtin_memcpy(&i, &f, 4);
+    return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
 
 // Everything the kernels need to know about one exposure of one patch.
 struct alignas(16) ExpDesc {
@@ -61,15 +75,17 @@ struct alignas(16) ExpDesc {
 };
 static_assert(sizeof(ExpDesc) == 128, "ExpDesc must be 128 bytes");
 
-// One wave tile: pixels [pix0, pix0 + npix) of exposure `exp`.
+// One wave tile: pixels [pix0, pix0 + npix) of exposure `exp`, with the
+// bounding box of their coordinates as order keys.
 struct alignas(8) TileDesc {
     int32_t exp;       // ExpDesc index
     int32_t npix;      // <= 64*K
     int32_t pix0;      // offset into the patch pixel arrays
     int32_t pad;
     int64_t part_off;  // first float of this tile's gradient partials
+    int32_t xlo, xhi, ylo, yhi;
 };
-static_assert(sizeof(TileDesc) == 24, "TileDesc must be 24 bytes");
+static_assert(sizeof(TileDesc) == 40, "TileDesc must be 40 bytes");
 
 // One reduce output: band `band` of a patch, its tiles [tile0, tile0+ntiles)
 // are contiguous and their partials contiguous from part0 (stride n_src*7).

@@ -22,6 +22,7 @@ struct EvalArgs {
     int persistent;
     int64_t grid;          // persistent: number of workgroups
     int precision;         // Precision mode
+    int cull;              // skip Gaussians whose cutoff box misses the tile
 };
 
 struct SplineView {

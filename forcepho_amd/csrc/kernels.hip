@@ -81,6 +81,15 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
     const M dFq = scale(mul(F, mul(dSq, F)), -1.0);
     const M dFp = scale(mul(F, mul(dSp, F)), -1.0);
 
+    // bounding box of the cutoff ellipse d^T F d <= 36: |dx| <= 6 sqrt(cov_xx),
+    // widened (0.1% + 0.01 pix) so that no pair the FP32 test accepts is
+    // outside it; skipping boxed-out tiles then changes no result bit.
+    const double hx = 6.0 * sqrt(cov.a) * 1.001 + 0.01;
+    const double hy = 6.0 * sqrt(cov.d) * 1.001 + 0.01;
+    g.xlo = order_key((float)(xc - hx));
+    g.xhi = order_key((float)(xc + hx));
+    g.ylo = order_key((float)(yc - hy));
+    g.yhi = order_key((float)(yc + hy));
     g.xcen = (float)xc;
     g.ycen = (float)yc;
     g.fxx = (float)F.a;
@@ -105,6 +114,7 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
     g.dy_dAlpha = CWp[2];
     g.dx_dDelta = CWp[1];
     g.dy_dDelta = CWp[3];
+    g.pad2[0] = g.pad2[1] = g.pad2[2] = g.pad2[3] = 0.0f;
 
     g64.xcen = xc;
     g64.ycen = yc;
@@ -150,6 +160,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// True when the Gaussian's cutoff box (gb = xlo, xhi, ylo, yhi keys) misses
+// the tile's pixel box tb; one 16-byte scalar load, integer compares only.
+__device__ __forceinline__ bool boxed_out(int cull, const int4 gb, const int4 tb) {
+    const int miss = (gb.y < tb.x ? 1 : 0) | (gb.x > tb.y ? 2 : 0) | (gb.w < tb.z ? 4 : 0) |
+                     (gb.z > tb.w ? 8 : 0);
+    return cull != 0 && miss != 0;
+}
+
 // Value of one image Gaussian at one pixel in FP64 (bright terms only).
 __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, float yp) {
     const double dx = (double)xp - g.xcen;
@@ -173,7 +191,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                                           const ImgGauss *__restrict__ gauss,
                                           const ImgGauss64 *__restrict__ gauss64,
                                           float *__restrict__ gpart,
-                                          double *__restrict__ cpart) {
+                                          double *__restrict__ cpart, int cull) {
     const int lane = threadIdx.x & (kWave - 1);
     const TileDesc t = tiles[tile];
     const ExpDesc &e = exps[t.exp];
@@ -205,8 +223,14 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         thr[k] = ie[k] > 0.0f ? kBrightSigma / ie[k] : __builtin_huge_valf();
     }
 
+    // A Gaussian whose cutoff box misses the tile's pixel box contributes
+    // nothing to any pixel of the tile (scalar test, no vector work).
+    const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
+#define FPHO_BOXED_OUT(G) boxed_out(cull, *reinterpret_cast<const int4 *>(&(G).xlo), tbox)
+
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
     for (int i = 0; i < ng; ++i) {
+        if (FPHO_BOXED_OUT(g[i])) continue;
         const float xc = g[i].xcen, yc = g[i].ycen;
         const float fxx = g[i].fxx, fyy = g[i].fyy, fxy = g[i].fxy;
         const float amp = g[i].amp, hc = g[i].hc;
@@ -262,8 +286,11 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     for (int s = 0; s < n_src; ++s) {
         float a0 = 0.f, sx = 0.f, sy = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f, a6 = 0.f;
         const int i0 = s * n_psf;
+        bool touched = false;
         for (int i = i0; i < i0 + n_psf; ++i) {
             const ImgGauss &G = g[i];
+            if (FPHO_BOXED_OUT(G)) continue;
+            touched = true;
             const float xc = G.xcen, yc = G.ycen;
             const float fxx = G.fxx, fyy = G.fyy, fxy = G.fxy;
             const float amp = G.amp, hc = G.hc;
@@ -296,6 +323,12 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                 }
             }
         }
+        float *o = out + kNParams * s;
+        if (!touched) {
+            // no Gaussian of this source reaches the tile
+            if (lane < kNParams) o[lane] = 0.0f;
+            continue;
+        }
         a0 = wave_sum(a0);
         sx = wave_sum(sx);
         sy = wave_sum(sy);
@@ -306,7 +339,6 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         if (lane == 0) {
             // the sky-position derivatives share the source's CW matrix
             const float *CW = e.CW + 4 * s;
-            float *o = out + kNParams * s;
             o[0] = a0;
             o[1] = CW[0] * sx + CW[2] * sy;  // dx/dalpha, dy/dalpha
             o[2] = CW[1] * sx + CW[3] * sy;  // dx/ddelta, dy/ddelta
@@ -316,6 +348,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             o[6] = a6;
         }
     }
+#undef FPHO_BOXED_OUT
 }
 
 // Non-persistent: wave w of workgroup b evaluates queue[b*W + w].
@@ -324,12 +357,12 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t waves_per_group, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart) {
+    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = blockIdx.x * waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart);
+    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull);
 }
 
 // Persistent: every wave pulls tiles from `counter` until the queue is empty.
@@ -339,7 +372,7 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t *__restrict__ counter, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart) {
+    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull) {
     const int lane = threadIdx.x & (kWave - 1);
     for (;;) {
         int slot = 0;
@@ -347,7 +380,7 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
         slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, kWave));
         if (slot >= n_tiles) break;
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart);
+        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull);
     }
 }
 
@@ -558,12 +591,13 @@ static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
         hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED>), dim3((unsigned)a.grid),
                            dim3(64 * W), 0, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
-                           a.counter, a.exps, a.gauss, a.gauss64, a.gpart, a.cpart);
+                           a.counter, a.exps, a.gauss, a.gauss64, a.gpart, a.cpart,
+                           (int32_t)a.cull);
     } else {
         const int64_t groups = (a.n_tiles + W - 1) / W;
         hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), 0,
                            stream, a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps,
-                           a.gauss, a.gauss64, a.gpart, a.cpart);
+                           a.gauss, a.gauss64, a.gpart, a.cpart, (int32_t)a.cull);
     }
     return hipGetLastError();
 }

@@ -90,6 +90,7 @@ fpho_launch default_launch(bool batch) {
     l.waves_per_group = 4;
     l.persistent = batch ? 1 : 0;
     l.groups_per_cu = 8;
+    l.cull = 1;
     return l;
 }
 
@@ -103,6 +104,7 @@ int check_launch(const fpho_launch &l) {
         return fail(FPHO_E_ARG, "persistent must be 0 or 1");
     if (l.groups_per_cu < 1 || l.groups_per_cu > 64)
         return fail(FPHO_E_ARG, "groups_per_cu must be in 1..64");
+    if (l.cull != 0 && l.cull != 1) return fail(FPHO_E_ARG, "cull must be 0 or 1");
     return FPHO_OK;
 }
 
@@ -158,6 +160,10 @@ struct fpho_patch {
     std::vector<int16_t> band_start, band_N;
     std::vector<int32_t> exposure_start, exposure_N, n_psf_per_source, psfgauss_start;
     std::vector<float> G_host;
+    // coordinate box of every 64-pixel chunk of every exposure (chunks start
+    // at the exposure start): xlo, xhi, ylo, yhi as order keys
+    std::vector<int64_t> chunk_start;  // [n_exp+1] first chunk of exposure e
+    std::vector<int32_t> chunk_box;    // [n_chunks][4]
     DevVec<float> data, ierr, xpix, ypix, residual;
     DevVec<float> rad2, D, crpix, crval, CW;
     DevVec<fpho_psf_gauss> psf;
@@ -338,6 +344,19 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
                 t.pix0 = p->exposure_start[e] + off;
                 t.pad = 0;
                 t.part_off = hp.n_part;
+                // union of the boxes of the tile's 64-pixel chunks
+                const int64_t c0 = p->chunk_start[e] + off / kWave;
+                const int64_t c1 = p->chunk_start[e] + (off + t.npix + kWave - 1) / kWave;
+                t.xlo = p->chunk_box[4 * c0 + 0];
+                t.xhi = p->chunk_box[4 * c0 + 1];
+                t.ylo = p->chunk_box[4 * c0 + 2];
+                t.yhi = p->chunk_box[4 * c0 + 3];
+                for (int64_t c = c0 + 1; c < c1; ++c) {
+                    t.xlo = std::min(t.xlo, p->chunk_box[4 * c + 0]);
+                    t.xhi = std::max(t.xhi, p->chunk_box[4 * c + 1]);
+                    t.ylo = std::min(t.ylo, p->chunk_box[4 * c + 2]);
+                    t.yhi = std::max(t.yhi, p->chunk_box[4 * c + 3]);
+                }
                 hp.n_part += (int64_t)ns * kNParams;
                 hp.tiles.push_back(t);
                 // cost model: both passes visit every (pixel, Gaussian) pair
@@ -435,6 +454,7 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
     a.gpart = plan.gpart.ptr;
     a.cpart = plan.cpart.ptr;
     a.pix_per_lane = plan.launch.pix_per_lane;
+    a.cull = plan.launch.cull;
     a.waves_per_group = plan.launch.waves_per_group;
     a.persistent = plan.launch.persistent;
     const int64_t groups_needed = (plan.n_tiles + a.waves_per_group - 1) / a.waves_per_group;
@@ -518,6 +538,24 @@ int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch 
     p->exposure_start.assign(d->exposure_start, d->exposure_start + d->n_exp);
     p->exposure_N.assign(d->exposure_N, d->exposure_N + d->n_exp);
     p->return_residual = return_residual != 0;
+    p->chunk_start.assign(1, 0);
+    for (int ex = 0; ex < d->n_exp; ++ex) {
+        const int64_t s0 = d->exposure_start[ex], n = d->exposure_N[ex];
+        for (int64_t c = 0; c < n; c += kWave) {
+            float xlo = d->xpix[s0 + c], xhi = xlo, ylo = d->ypix[s0 + c], yhi = ylo;
+            for (int64_t i = s0 + c; i < s0 + std::min<int64_t>(n, c + kWave); ++i) {
+                xlo = std::min(xlo, d->xpix[i]);
+                xhi = std::max(xhi, d->xpix[i]);
+                ylo = std::min(ylo, d->ypix[i]);
+                yhi = std::max(yhi, d->ypix[i]);
+            }
+            p->chunk_box.push_back(order_key(xlo));
+            p->chunk_box.push_back(order_key(xhi));
+            p->chunk_box.push_back(order_key(ylo));
+            p->chunk_box.push_back(order_key(yhi));
+        }
+        p->chunk_start.push_back((int64_t)p->chunk_box.size() / 4);
+    }
     hipError_t e = hipSuccess;
     if ((e = p->data.upload(d->data, d->npix)) != hipSuccess ||
         (e = p->ierr.upload(d->ierr, d->npix)) != hipSuccess ||

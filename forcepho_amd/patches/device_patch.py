@@ -112,9 +112,10 @@ class HIPPatchMixin(DevicePatchMixin):
         return HIPProposer()
 
     # -- descriptors -------------------------------------------------------
-    def _desc(self):
+    def _desc(self, meta_only=False):
         """fpho_patch_desc over this patch's host arrays (kept alive in
-        ``self._desc_keep`` until the call returns)."""
+        ``self._desc_keep`` until the call returns).  With ``meta_only`` the
+        pixel arrays are left out (the library ignores them then)."""
         keep = {}
 
         def arr(name, dtype, ctype):
@@ -124,10 +125,11 @@ class HIPPatchMixin(DevicePatchMixin):
 
         d = _lib.PatchDesc()
         d.npix = int(self.npix)
-        d.data = arr("data", np.float32, ctypes.c_float)
-        d.ierr = arr("ierr", np.float32, ctypes.c_float)
-        d.xpix = arr("xpix", np.float32, ctypes.c_float)
-        d.ypix = arr("ypix", np.float32, ctypes.c_float)
+        if not meta_only:
+            d.data = arr("data", np.float32, ctypes.c_float)
+            d.ierr = arr("ierr", np.float32, ctypes.c_float)
+            d.xpix = arr("xpix", np.float32, ctypes.c_float)
+            d.ypix = arr("ypix", np.float32, ctypes.c_float)
         d.n_bands = int(len(self.band_N))
         d.n_exp = int(len(self.exposure_N))
         d.band_start = arr("band_start", np.int16, ctypes.c_int16)
@@ -173,7 +175,7 @@ class HIPPatchMixin(DevicePatchMixin):
 
     def replace_device_meta_ptrs(self):
         """Upload the currently packed metadata (device_patch.py:302-311)."""
-        d = self._desc()
+        d = self._desc(meta_only=True)
         _lib.check(_lib.lib().fpho_patch_set_meta(self.device_patch, ctypes.byref(d)))
         self._desc_keep = None
 
@@ -199,9 +201,15 @@ class HIPPatchMixin(DevicePatchMixin):
                                               _lib.fptr(out, ctypes.c_float)))
         return out
 
-    def set_launch(self, pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8):
-        l = _lib.Launch(pix_per_lane, waves_per_group, persistent, groups_per_cu)
+    def set_launch(self, pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8,
+                   cull=1):
+        l = _lib.Launch(pix_per_lane, waves_per_group, persistent, groups_per_cu, cull)
         _lib.check(_lib.lib().fpho_patch_set_launch(self.device_patch, ctypes.byref(l)))
+
+    def set_precision(self, mode):
+        """"fp32", "fp64_reduce" or "mixed" (default)."""
+        _lib.check(_lib.lib().fpho_patch_set_precision(self.device_patch,
+                                                       _lib.PRECISION[mode]))
 
     def free(self):
         handle = getattr(self, "device_patch", None)

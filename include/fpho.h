@@ -122,6 +122,8 @@ typedef struct fpho_launch {
     int32_t waves_per_group;  /* workgroup = 64*waves_per_group lanes (1..4)*/
     int32_t persistent;       /* 1: persistent waves pull tiles from a queue*/
     int32_t groups_per_cu;    /* persistent grid = groups_per_cu * n_CU     */
+    int32_t cull;             /* 1: skip (tile, Gaussian) pairs whose cutoff
+                                 box misses the tile (results unchanged)    */
 } fpho_launch;
 
 /* ---------------------------------------------------------------- runtime */

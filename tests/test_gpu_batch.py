@@ -83,3 +83,19 @@ def test_precision_modes(patches, mode):
     print(f"precision {mode}: worst chi2 rel err {worst:.2e}", flush=True)
     assert worst <= (CHI2_RTOL if mode != "fp32" else 1e-5)
     batch.free()
+
+
+def test_box_culling_changes_no_bit(patches):
+    """Skipping (tile, Gaussian) pairs whose cutoff box misses the tile gives
+    results identical to evaluating every pair."""
+    props = [proposal_for(sc, th) for _, sc, th in patches]
+    batch = PatchBatch([p for p, _, _ in patches])
+    for mode in ("mixed", "fp64_reduce"):
+        batch.set_precision(mode)
+        batch.set_launch(cull=1)
+        c1, d1 = _batch_eval(batch, props)
+        batch.set_launch(cull=0)
+        c0, d0 = _batch_eval(batch, props)
+        for a, b in zip(c1 + d1, c0 + d0):
+            assert np.array_equal(a, b), mode
+    batch.free()

@@ -81,3 +81,14 @@ def test_determinism(c2):
     a = patch.get_proposer().evaluate_proposal(prop, patch=patch)
     b = patch.get_proposer().evaluate_proposal(prop, patch=patch)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_config2_culling_bitwise(c2):
+    patch, scene, theta = c2
+    prop = proposal_for(scene, synth.make_proposals(theta, 8, 1, config=2, patch_id=5)[0])
+    patch.set_launch(cull=1)
+    a = patch.get_proposer().evaluate_proposal(prop, patch=patch)
+    patch.set_launch(cull=0)
+    b = patch.get_proposer().evaluate_proposal(prop, patch=patch)
+    patch.set_launch(cull=1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
