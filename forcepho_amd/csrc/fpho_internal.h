@@ -31,25 +31,40 @@ constexpr int kWave = 64;
 constexpr int kNParams = FPHO_NPARAMS;
 
 // One image-plane Gaussian (a source Sersic component convolved with one PSF
-// component) plus the Jacobian of its 6 image parameters with respect to the
-// 7 on-sky source parameters.  Same quantities as ImageGaussian in
-// forcepho/src/common_kernel.cc:20-46, re-laid-out for scalar loads:
+// component) in the form the pixel pass consumes.  It carries the same
+// information as ImageGaussian in forcepho/src/common_kernel.cc:20-46 (value
+// parameters and the Jacobian to the 7 on-sky parameters), re-expressed to
+// minimise vector instructions per (pixel, Gaussian) pair:
+//
+//   with s = -log2(e)/2, the pass uses v' = s F d (d = pixel - centre), so
+//   that E' = d.v' = s E is directly the exp2 argument (exp(-E/2) =
+//   exp2(E')), and 24 s^2 H = hcs + v'.v' with hcs = s^2 (24 - Fxx - Fyy);
+//   the value is C = ampA exp2(E') (hcs + v'.v') with ampA = amp/(24 s^2).
+//
+//   The gradient pass accumulates, per Gaussian and lane, the 12 pixel sums
+//   of the monomials the derivative terms are made of (see eval_tile) and
+//   converts them once per Gaussian with the constants below, which fold in
+//   amp, the powers of s and the Jacobian (dA/dflux, dA/dq, dF/dq, ...).
+//
 //   box   : bounding box of the cutoff ellipse (exp argument <= 36, widened by
-//           a safety margin), as order-preserving integer keys of the float
-//           coordinates, so the per-tile skip test runs on the scalar ALU;
-//   hot   : what the residual pass needs; hc = 1 - (fxx + fyy)/24 folds the
-//           constant part of the pixel-integral correction H;
-//   cold  : the Jacobian, read by the gradient pass.
+//           a safety margin), as order-preserving integer keys, so the
+//           per-tile skip test runs on the scalar ALU.
 struct alignas(16) ImgGauss {
-    int32_t xlo, xhi, ylo, yhi;                 // box (16 bytes)
-    float xcen, ycen, fxx, fyy, fxy, amp, hc, pad0;   // hot (32 bytes)
-    float dA_dFlux, dA_dQ, dFxx_dQ, dFyy_dQ;    // cold (64 bytes)
-    float dFxy_dQ, dA_dPA, dFxx_dPA, dFyy_dPA;
-    float dFxy_dPA, dA_dSersic, dA_drh, pad1;
-    float dx_dAlpha, dy_dAlpha, dx_dDelta, dy_dDelta;
-    float pad2[4];
+    int32_t xlo, xhi, ylo, yhi;              // box
+    float xcen, ycen, sfxx, sfyy;            // value parameters (hot)
+    float sfxy, hcs, ampA, peak;             // peak: bound on |C| over all pixels
+    // gradient epilogue constants (cold)
+    float cA_flux, cA_q, cA_pa, cA_n;        // dA/d(.) / (24 s^2)
+    float cA_rh, k_axx, k_axy, k_e;          // -amp/(48s^2), -amp/(24s^2), -amp/24
+    float k_u, k_x1, k_x2, dFxx_dQ;          // amp/(12s), amp/(24s^3), -amp/(12s^2)
+    float dFyy_dQ, dFxy_dQ, dFxx_dPA, dFyy_dPA;
+    float dFxy_dPA, pad1, pad2, pad3;
 };
 static_assert(sizeof(ImgGauss) == 128, "ImgGauss must be 128 bytes");
+
+// s = -log2(e)/2 and the cutoff exp argument 36 in scaled units (36 s).
+constexpr double kScaleS = -0.72134752044448170368;
+constexpr float kCutScaled = (float)(36.0 * kScaleS);
 
 // Order-preserving map of a float to a signed integer (a < b <=> key(a) <
 // key(b) for all non-NaN floats), so ranges compare with integer instructions.

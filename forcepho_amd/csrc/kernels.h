@@ -37,6 +37,7 @@ struct SplineView {
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
                           ImgGauss *out, ImgGauss64 *out64, hipStream_t stream);
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
+hipError_t launch_count_pairs(const EvalArgs &a, unsigned long long *count, hipStream_t stream);
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,
                          const double *cpart, double *chi2, double *dchi2, int fp64,
                          hipStream_t stream);

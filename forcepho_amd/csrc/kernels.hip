@@ -90,31 +90,45 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
     g.xhi = order_key((float)(xc + hx));
     g.ylo = order_key((float)(yc - hy));
     g.yhi = order_key((float)(yc + hy));
+
+    // value parameters in the scaled variables of the pixel pass
+    const double sc = kScaleS, sc2 = sc * sc;
     g.xcen = (float)xc;
     g.ycen = (float)yc;
-    g.fxx = (float)F.a;
-    g.fyy = (float)F.d;
-    g.fxy = (float)F.c;
-    g.amp = (float)amp;
-    g.hc = (float)(1.0 - (F.a + F.d) * (1.0 / 24.0));
-    g.pad0 = 0.0f;
-    g.dA_dQ = (float)(0.5 * amp * trace(mul(cov, dFq)));
-    g.dA_dPA = (float)(0.5 * amp * trace(mul(cov, dFp)));
-    g.dA_dFlux = (float)(tmp * am);
-    g.dA_dSersic = (float)(tmp * flux * src.damplitude_dnsersic[bin]);
-    g.dA_drh = (float)(tmp * flux * src.damplitude_drh[bin]);
+    g.sfxx = (float)(sc * F.a);
+    g.sfyy = (float)(sc * F.d);
+    g.sfxy = (float)(sc * F.c);
+    g.hcs = (float)(sc2 * (24.0 - F.a - F.d));
+    g.ampA = (float)(amp / (24.0 * sc2));
+    // |C| = |ampA| gp h' <= |ampA| (|hcs| + s^2 tr(F) max_E E e^{-E/2}), since
+    // |v'|^2 <= s^2 lambda_max(F) E and E e^{-E/2} <= 2/e; widened by 1%
+    g.peak = (float)(1.01 * fabs(amp / (24.0 * sc2)) *
+                     (fabs(sc2 * (24.0 - F.a - F.d)) + sc2 * (F.a + F.d) * 0.73575888234288467));
+
+    // gradient epilogue constants: Jacobian of (A, F) with respect to the
+    // source parameters folded with amp and the powers of s (derivation in
+    // eval_tile)
+    const double dA_dq = 0.5 * amp * trace(mul(cov, dFq));
+    const double dA_dpa = 0.5 * amp * trace(mul(cov, dFp));
+    const double c24 = 1.0 / (24.0 * sc2);
+    g.cA_flux = (float)(tmp * am * c24);
+    g.cA_q = (float)(dA_dq * c24);
+    g.cA_pa = (float)(dA_dpa * c24);
+    g.cA_n = (float)(tmp * flux * src.damplitude_dnsersic[bin] * c24);
+    g.cA_rh = (float)(tmp * flux * src.damplitude_drh[bin] * c24);
+    g.k_axx = (float)(-amp / (48.0 * sc2));
+    g.k_axy = (float)(-amp / (24.0 * sc2));
+    g.k_e = (float)(-amp / 24.0);
+    g.k_u = (float)(amp / (12.0 * sc));
+    g.k_x1 = (float)(amp / (24.0 * sc2 * sc));
+    g.k_x2 = (float)(-amp / (12.0 * sc2));
     g.dFxx_dQ = (float)dFq.a;
     g.dFyy_dQ = (float)dFq.d;
     g.dFxy_dQ = (float)dFq.c;
     g.dFxx_dPA = (float)dFp.a;
     g.dFyy_dPA = (float)dFp.d;
     g.dFxy_dPA = (float)dFp.c;
-    g.pad1 = 0.0f;
-    g.dx_dAlpha = CWp[0];
-    g.dy_dAlpha = CWp[2];
-    g.dx_dDelta = CWp[1];
-    g.dy_dDelta = CWp[3];
-    g.pad2[0] = g.pad2[1] = g.pad2[2] = g.pad2[3] = 0.0f;
+    g.pad1 = g.pad2 = g.pad3 = 0.0f;
 
     g64.xcen = xc;
     g64.ycen = yc;
@@ -145,8 +159,6 @@ __global__ __launch_bounds__(128) void convert_kernel(
 
 // --------------------------------------------------------------- pixel pass
 
-// exp(-E/2) as one hardware exp2: exp2(E * (-log2(e)/2)).
-constexpr float kHalfLog2e = -0.72134752044448170368f;
 constexpr float kMaskedCoord = 1.0e6f;  // lanes past the tile end: far away
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -185,6 +197,21 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 //                  exceeds kBrightSigma * sigma of its pixel is re-evaluated
 //                  in FP64 (a wave branch taken only near bright cores).
 //   MIXED = false: FP32 residual, as forcepho/src/common_kernel.cc:57-83.
+//
+// Scaled variables (ImgGauss): d = pixel - centre, v' = s F d, E' = d.v' =
+// s E, gp = exp(-E/2) = exp2(E'), h' = hcs + v'.v' = 24 s^2 H.
+// Gradient (common_kernel.cc:95-143): with e = -2 r ierr^2 gp and
+// a = e H = a' / (24 s^2), a' = e h', the reference's per-pair terms are
+//   dC/dA   = a
+//   dC/dx   = amp (a vx - e (F v)_x / 12)
+//   dC/dFxx = amp (-a dx^2 / 2 - e / 24 + e dx vx / 12)
+//   dC/dFyy = amp (-a dy^2 / 2 - e / 24 + e dy vy / 12)
+//   dC/dFxy = amp (-a dx dy + e (dy vx + dx vy) / 12)
+// With v = v'/s, F v = (s F v')/s^2, dx vx = u'/s (u' = dx v'x), they are
+// linear in 12 pixel sums of a', a' v'x, e v'x, a' v'y, e v'y,
+// a' dx^2, a' dy^2, a' dx dy, e, e u', e w' (w' = dy v'y), e cr'
+// (cr' = dy v'x + dx v'y); each sum costs one FMA per pair and is turned
+// into the 7 source partials once per Gaussian (constants in ImgGauss).
 template <int K, bool MIXED>
 __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__ tiles,
                                           const ExpDesc *__restrict__ exps,
@@ -228,25 +255,37 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
 #define FPHO_BOXED_OUT(G) boxed_out(cull, *reinterpret_cast<const int4 *>(&(G).xlo), tbox)
 
+    // smallest bright-term threshold over the tile: a Gaussian whose peak
+    // stays below it cannot have a bright term here
+    float tmin = thr[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) tmin = fminf(tmin, thr[k]);
+    if (MIXED) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, off, kWave));
+        tmin = __builtin_amdgcn_readfirstlane(tmin);
+    }
+
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
     for (int i = 0; i < ng; ++i) {
         if (FPHO_BOXED_OUT(g[i])) continue;
         const float xc = g[i].xcen, yc = g[i].ycen;
-        const float fxx = g[i].fxx, fyy = g[i].fyy, fxy = g[i].fxy;
-        const float amp = g[i].amp, hc = g[i].hc;
+        const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
+        const float hcs = g[i].hcs, ampA = g[i].ampA;
+        const bool maybe_bright = g[i].peak > tmin;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const float dx = x[k] - xc;
             const float dy = y[k] - yc;
-            const float vx = fxx * dx + fxy * dy;
-            const float vy = fyy * dy + fxy * dx;
-            const float ea = dx * vx + dy * vy;
-            if (!(ea > FPHO_MAX_EXP_ARG)) {
-                const float gp = __builtin_amdgcn_exp2f(ea * kHalfLog2e);
-                const float h = hc + (vx * vx + vy * vy) * (1.0f / 24.0f);
-                const float c = amp * gp * h;
+            const float vx = sfxx * dx + sfxy * dy;
+            const float vy = sfyy * dy + sfxy * dx;
+            const float ea = dx * vx + dy * vy;      // s E
+            if (!(ea < kCutScaled)) {                 // E <= 36 (cutoff)
+                const float gp = __builtin_amdgcn_exp2f(ea);
+                const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
+                const float c = ampA * gp * h;
                 if (MIXED) {
-                    if (fabsf(c) > thr[k])
+                    if (maybe_bright && fabsf(c) > thr[k])
                         rd[k] -= pair_value64(g64[i], x[k], y[k]);
                     else
                         rd[k] -= (double)c;
@@ -284,7 +323,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     // ---- pass 2: per source, 7 partials   (common_kernel.cc:95-143)
     float *__restrict__ out = gpart + t.part_off;
     for (int s = 0; s < n_src; ++s) {
-        float a0 = 0.f, sx = 0.f, sy = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f, a6 = 0.f;
+        float p0 = 0.f, sx = 0.f, sy = 0.f, p3 = 0.f, p4 = 0.f, p5 = 0.f, p6 = 0.f;
         const int i0 = s * n_psf;
         bool touched = false;
         for (int i = i0; i < i0 + n_psf; ++i) {
@@ -292,36 +331,64 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             if (FPHO_BOXED_OUT(G)) continue;
             touched = true;
             const float xc = G.xcen, yc = G.ycen;
-            const float fxx = G.fxx, fyy = G.fyy, fxy = G.fxy;
-            const float amp = G.amp, hc = G.hc;
+            const float sfxx = G.sfxx, sfyy = G.sfyy, sfxy = G.sfxy, hcs = G.hcs;
+            // the 12 per-Gaussian pixel sums (over this lane's K pixels)
+            float PA = 0.f, X1 = 0.f, X2 = 0.f, Y1 = 0.f, Y2 = 0.f, Pxx = 0.f;
+            float Pyy = 0.f, Pxy = 0.f, Pe = 0.f, Peu = 0.f, Pew = 0.f, Pecr = 0.f;
+            bool any = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const float dx = x[k] - xc;
                 const float dy = y[k] - yc;
-                const float vx = fxx * dx + fxy * dy;
-                const float vy = fyy * dy + fxy * dx;
-                const float ea = dx * vx + dy * vy;
-                if (!(ea > FPHO_MAX_EXP_ARG)) {
-                    const float gp = __builtin_amdgcn_exp2f(ea * kHalfLog2e);
-                    const float h = hc + (vx * vx + vy * vy) * (1.0f / 24.0f);
-                    float dC_dA = rr[k] * gp;
-                    const float c_h = dC_dA * amp;
-                    dC_dA *= h;
-                    const float C = c_h * h;
-                    const float dC_dx = C * vx - c_h * (fxx * vx + fxy * vy) * (1.0f / 12.0f);
-                    const float dC_dy = C * vy - c_h * (fyy * vy + fxy * vx) * (1.0f / 12.0f);
-                    const float dC_dfx = -0.5f * C * dx * dx - c_h * (1.0f - 2.0f * dx * vx) * (1.0f / 24.0f);
-                    const float dC_dfy = -0.5f * C * dy * dy - c_h * (1.0f - 2.0f * dy * vy) * (1.0f / 24.0f);
-                    const float dC_dfxy = -C * dx * dy + c_h * (dy * vx + dx * vy) * (1.0f / 12.0f);
-                    a0 += G.dA_dFlux * dC_dA;
-                    sx += dC_dx;
-                    sy += dC_dy;
-                    a3 += G.dA_dQ * dC_dA + dC_dfx * G.dFxx_dQ + dC_dfxy * G.dFxy_dQ + dC_dfy * G.dFyy_dQ;
-                    a4 += G.dA_dPA * dC_dA + dC_dfx * G.dFxx_dPA + dC_dfxy * G.dFxy_dPA + dC_dfy * G.dFyy_dPA;
-                    a5 += G.dA_dSersic * dC_dA;
-                    a6 += G.dA_drh * dC_dA;
+                const float vx = sfxx * dx + sfxy * dy;
+                const float vy = sfyy * dy + sfxy * dx;
+                const float u = dx * vx;
+                const float ea = fmaf(dy, vy, u);
+                const bool in = !(ea < kCutScaled);
+                if (__builtin_amdgcn_ballot_w64(in) == 0) continue;
+                any = true;
+                if (in) {
+                    const float gp = __builtin_amdgcn_exp2f(ea);
+                    const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
+                    const float ee = rr[k] * gp;
+                    const float a = ee * h;
+                    PA += a;
+                    X1 = fmaf(a, vx, X1);
+                    X2 = fmaf(ee, vx, X2);   // sum e v'x (s F applied below)
+                    Y1 = fmaf(a, vy, Y1);
+                    Y2 = fmaf(ee, vy, Y2);   // sum e v'y
+                    const float ax = a * dx;
+                    const float ay = a * dy;
+                    Pxx = fmaf(ax, dx, Pxx);
+                    Pxy = fmaf(ax, dy, Pxy);
+                    Pyy = fmaf(ay, dy, Pyy);
+                    Pe += ee;
+                    Peu = fmaf(ee, u, Peu);
+                    Pew = fmaf(ee, ea - u, Pew);
+                    Pecr = fmaf(ee, fmaf(dy, vx, dx * vy), Pecr);
                 }
             }
+            if (!any) continue;
+            // per-Gaussian epilogue: image-space derivative sums -> 7 partials
+            const float gfx = G.k_axx * Pxx + G.k_e * Pe + G.k_u * Peu;
+            const float gfy = G.k_axx * Pyy + G.k_e * Pe + G.k_u * Pew;
+            const float gfxy = G.k_axy * Pxy + G.k_u * Pecr;
+            p0 = fmaf(G.cA_flux, PA, p0);
+            p5 = fmaf(G.cA_n, PA, p5);
+            p6 = fmaf(G.cA_rh, PA, p6);
+            p3 = fmaf(G.cA_q, PA, p3);
+            p3 = fmaf(G.dFxx_dQ, gfx, p3);
+            p3 = fmaf(G.dFyy_dQ, gfy, p3);
+            p3 = fmaf(G.dFxy_dQ, gfxy, p3);
+            p4 = fmaf(G.cA_pa, PA, p4);
+            p4 = fmaf(G.dFxx_dPA, gfx, p4);
+            p4 = fmaf(G.dFyy_dPA, gfy, p4);
+            p4 = fmaf(G.dFxy_dPA, gfxy, p4);
+            // sum e (sF v')_x = sfxx sum e v'x + sfxy sum e v'y (and for y)
+            const float fX = sfxx * X2 + sfxy * Y2;
+            const float fY = sfyy * Y2 + sfxy * X2;
+            sx = fmaf(G.k_x1, X1, fmaf(G.k_x2, fX, sx));
+            sy = fmaf(G.k_x1, Y1, fmaf(G.k_x2, fY, sy));
         }
         float *o = out + kNParams * s;
         if (!touched) {
@@ -329,23 +396,23 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             if (lane < kNParams) o[lane] = 0.0f;
             continue;
         }
-        a0 = wave_sum(a0);
+        p0 = wave_sum(p0);
         sx = wave_sum(sx);
         sy = wave_sum(sy);
-        a3 = wave_sum(a3);
-        a4 = wave_sum(a4);
-        a5 = wave_sum(a5);
-        a6 = wave_sum(a6);
+        p3 = wave_sum(p3);
+        p4 = wave_sum(p4);
+        p5 = wave_sum(p5);
+        p6 = wave_sum(p6);
         if (lane == 0) {
             // the sky-position derivatives share the source's CW matrix
             const float *CW = e.CW + 4 * s;
-            o[0] = a0;
+            o[0] = p0;
             o[1] = CW[0] * sx + CW[2] * sy;  // dx/dalpha, dy/dalpha
             o[2] = CW[1] * sx + CW[3] * sy;  // dx/ddelta, dy/ddelta
-            o[3] = a3;
-            o[4] = a4;
-            o[5] = a5;
-            o[6] = a6;
+            o[3] = p3;
+            o[4] = p4;
+            o[5] = p5;
+            o[6] = p6;
         }
     }
 #undef FPHO_BOXED_OUT
@@ -382,6 +449,46 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
         eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull);
     }
+}
+
+// Number of (pixel, image Gaussian) pairs inside the cutoff (exp argument
+// <= 36) over the tiles, with the pixel pass's own FP32 test.  Measurement
+// only: it gives the reference algorithm's FLOP count for the roofline.
+template <int K>
+__global__ __launch_bounds__(256) void count_pairs_kernel(
+    const TileDesc *__restrict__ tiles, int32_t n_tiles, int32_t waves_per_group,
+    const ExpDesc *__restrict__ exps, const ImgGauss *__restrict__ gauss,
+    unsigned long long *__restrict__ count) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = blockIdx.x * waves_per_group + w;
+    if (tile >= n_tiles) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const TileDesc t = tiles[tile];
+    const ExpDesc &e = exps[t.exp];
+    const ImgGauss *__restrict__ g = gauss + e.gauss_off;
+    const int ng = e.n_src * e.n_psf;
+    float x[K], y[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = k * kWave + lane;
+        x[k] = p < t.npix ? e.xpix[t.pix0 + p] : kMaskedCoord;
+        y[k] = p < t.npix ? e.ypix[t.pix0 + p] : kMaskedCoord;
+    }
+    const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
+    unsigned long long n = 0;
+    for (int i = 0; i < ng; ++i) {
+        if (boxed_out(1, *reinterpret_cast<const int4 *>(&g[i].xlo), tbox)) continue;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float dx = x[k] - g[i].xcen;
+            const float dy = y[k] - g[i].ycen;
+            const float vx = g[i].sfxx * dx + g[i].sfxy * dy;
+            const float vy = g[i].sfyy * dy + g[i].sfxy * dx;
+            const float ea = dx * vx + dy * vy;
+            n += __popcll(__builtin_amdgcn_ballot_w64(!(ea < kCutScaled)));
+        }
+    }
+    if (lane == 0) atomicAdd(count, n);
 }
 
 __global__ void zero_counter_kernel(int32_t *counter) {
@@ -617,6 +724,34 @@ hipError_t launch_eval(const EvalArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;
     return a.precision == kPrecMixed ? launch_eval_p<true>(a, stream)
                                      : launch_eval_p<false>(a, stream);
+}
+
+hipError_t launch_count_pairs(const EvalArgs &a, unsigned long long *count,
+                              hipStream_t stream) {
+    if (a.n_tiles <= 0) return hipSuccess;
+    const int W = a.waves_per_group;
+    const unsigned groups = (unsigned)((a.n_tiles + W - 1) / W);
+    switch (a.pix_per_lane) {
+        case 1:
+            hipLaunchKernelGGL(count_pairs_kernel<1>, dim3(groups), dim3(64 * W), 0, stream,
+                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
+            break;
+        case 2:
+            hipLaunchKernelGGL(count_pairs_kernel<2>, dim3(groups), dim3(64 * W), 0, stream,
+                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
+            break;
+        case 4:
+            hipLaunchKernelGGL(count_pairs_kernel<4>, dim3(groups), dim3(64 * W), 0, stream,
+                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
+            break;
+        case 8:
+            hipLaunchKernelGGL(count_pairs_kernel<8>, dim3(groups), dim3(64 * W), 0, stream,
+                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,

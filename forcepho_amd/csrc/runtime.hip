@@ -437,11 +437,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     return FPHO_OK;
 }
 
-int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
-             hipStream_t stream) {
-    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
-    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
-                           stream));
+EvalArgs eval_args(Plan &plan) {
     EvalArgs a;
     a.tiles = plan.tiles.ptr;
     a.queue = plan.queue.ptr;
@@ -461,6 +457,15 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
     a.grid = std::min<int64_t>(groups_needed,
                                (int64_t)plan.launch.groups_per_cu * cu_count_cached());
     if (a.grid < 1) a.grid = 1;
+    return a;
+}
+
+int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
+             hipStream_t stream) {
+    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
+    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
+                           stream));
+    const EvalArgs a = eval_args(plan);
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[1], stream));
     HIP_TRY(launch_eval(a, stream));
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[2], stream));
@@ -787,6 +792,22 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
     HIP_TRY(hipEventSynchronize(b->plan.ev[3]));
     for (int i = 0; i < 3; ++i)
         HIP_TRY(hipEventElapsedTime(&ms3[i], b->plan.ev[i], b->plan.ev[i + 1]));
+    return FPHO_OK;
+}
+
+int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *props, int64_t *n_in) {
+    if (!b || !props || !n_in) return fail(FPHO_E_ARG, "null argument");
+    int rc = ensure_batch_plan(b);
+    if (rc) return rc;
+    DevVec<unsigned long long> cnt;
+    HIP_TRY(cnt.alloc(1));
+    HIP_TRY(hipMemset(cnt.ptr, 0, sizeof(unsigned long long)));
+    HIP_TRY(launch_convert(b->plan.exps.ptr, b->plan.n_exp, props, b->plan.gauss.ptr,
+                           b->plan.gauss64.ptr, nullptr));
+    HIP_TRY(launch_count_pairs(eval_args(b->plan), cnt.ptr, nullptr));
+    unsigned long long h = 0;
+    HIP_TRY(hipMemcpy(&h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    *n_in = (int64_t)h;
     return FPHO_OK;
 }
 

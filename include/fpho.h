@@ -192,6 +192,11 @@ int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *proposals_dev,
  * and returns milliseconds of convert, pixel pass and reduce.  Timing adds
  * event records to the stream; leave it off when capturing graphs. */
 int fpho_batch_set_timing(fpho_batch *b, int on);
+/* Measurement: number of (pixel, image Gaussian) pairs inside the cutoff
+ * (exp argument <= 36) for the given proposals (device), counted with the
+ * pixel pass's own FP32 test.  Synchronous; allocates a counter. */
+int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *proposals_dev,
+                           int64_t *n_in);
 int fpho_batch_last_timing(fpho_batch *b, float *ms3);
 int fpho_batch_free(fpho_batch *b);
 
