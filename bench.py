@@ -28,6 +28,7 @@ METRIC = ("lnlike+grad evals/sec & Gaussian-pixel evals/s, batched vs plain, "
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
 FLOP_PER_PAIR = 98         # algorithmic FLOPs per (pixel, image Gaussian) pair
 FLOP_PER_CUT_PAIR = 11     # dx, dy, v, exp argument of a pair beyond the cutoff
+TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "pixel_pass_config4_traffic.txt")
 
 
 def log(*a):
@@ -67,6 +68,23 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 5}), flush=True)
+
+
+def measured_traffic(launch):
+    """HBM bytes per pixel-pass launch from the committed rocprofv3
+    byte-counter summary, if it was taken on this kernel source and launch
+    shape; else (None, reason)."""
+    from forcepho_amd import profiling
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None, "no byte-counter summary"
+    kv = profiling.read_summary(TRAFFIC_SUMMARY)
+    shape = ",".join(str(launch[k]) for k in ("pix_per_lane", "waves_per_group",
+                                              "persistent", "groups_per_cu"))
+    if kv.get("kernel_source_sha256_16") != profiling.kernel_source_digest():
+        return None, "byte-counter summary taken on other kernel source"
+    if kv.get("launch") != shape:
+        return None, f"byte-counter summary taken at launch {kv.get('launch')}, not {shape}"
+    return float(kv["traffic_bytes_per_launch"]), os.path.relpath(TRAFFIC_SUMMARY, HERE)
 
 
 def algorithmic_counts(specs):
@@ -187,6 +205,7 @@ def main():
     else:
         launch = load_or_default("batch")
     batch.set_launch(**launch)
+    log(f"launch {launch}")
     splines = DeviceSplines()
     thetas = [synth.make_proposals(th, len(p.band_N), 1, config=4, patch_id=i)[0]
               for i, (p, sc, th) in enumerate(patches)]
@@ -290,6 +309,7 @@ def main():
                    "pairs_per_s": c["pairs_per_s"]}
             log(f"cpu reference: {c['pairs_per_s']:.3e} pairs/s on {c['patches']} patches")
 
+    traffic, traffic_src = measured_traffic(launch)
     out = {
         "metric": METRIC, "value": evals_per_s, "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_batched * 1e3,
@@ -304,7 +324,8 @@ def main():
         "batched_over_plain": (evals_per_s / plain) if plain else None,
         "chi2_rel_err_sample": chi2_err,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "eval pixel pass (eval_kernel / eval_persistent_kernel)",
                      "algorithmic_flop_per_sweep": flops_alg,
                      "pairs_in_cutoff": n_in,

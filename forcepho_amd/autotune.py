@@ -4,7 +4,7 @@ Candidates are (pixels per lane K, waves per workgroup W, persistent or
 grid launch, workgroups per CU for the persistent grid).  Each candidate is
 timed with HIP events on the evaluation stream over a few repetitions of a
 representative batch; the fastest is stored in a JSON cache keyed by device
-name and workload kind, which ``load_or_default`` reads at start-up.  Results
+architecture and workload kind, which ``load_or_default`` reads at start-up.  Results
 do not depend on the persistent/grid choice or on W (same tiles, same fixed
 reduction order); K changes the tiling and therefore only the rounding of
 the FP32 partial sums.
@@ -36,9 +36,11 @@ def candidates(kind="batch"):
 
 
 def _device_name():
+    """Cache key: the GPU architecture (e.g. gfx950), which fixes the best
+    launch shape; marketing names differ between tools on one device."""
     try:
         import torch
-        return torch.cuda.get_device_name(0)
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     except Exception:
         return "unknown"
 

@@ -23,7 +23,11 @@ struct EvalArgs {
     int64_t grid;          // persistent: number of workgroups
     int precision;         // Precision mode
     int cull;              // skip Gaussians whose cutoff box misses the tile
+    int mask_cap;          // Gaussians per exposure with an LDS slot mask (<= kMaskCapMax)
 };
+
+// LDS bytes per wave for the pass-1 slot masks, at most (4 waves: 32 KiB).
+constexpr int kMaskCapMax = 8192;
 
 struct SplineView {
     const double *coeffs;  // [n_amp][n_tx-4][n_ty-4]

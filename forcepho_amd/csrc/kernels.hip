@@ -212,13 +212,18 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 // a' dx^2, a' dy^2, a' dx dy, e, e u', e w' (w' = dy v'y), e cr'
 // (cr' = dy v'x + dx v'y); each sum costs one FMA per pair and is turned
 // into the 7 source partials once per Gaussian (constants in ImgGauss).
+//
+// Pass 1 records, per Gaussian, which of the K pixel slots hold a pixel inside
+// the cutoff (a K-bit mask, one byte in LDS per Gaussian for the first
+// mask_cap Gaussians); pass 2 evaluates only those slots.
 template <int K, bool MIXED>
 __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__ tiles,
                                           const ExpDesc *__restrict__ exps,
                                           const ImgGauss *__restrict__ gauss,
                                           const ImgGauss64 *__restrict__ gauss64,
                                           float *__restrict__ gpart,
-                                          double *__restrict__ cpart, int cull) {
+                                          double *__restrict__ cpart, int cull,
+                                          uint8_t *__restrict__ amask, int mask_cap) {
     const int lane = threadIdx.x & (kWave - 1);
     const TileDesc t = tiles[tile];
     const ExpDesc &e = exps[t.exp];
@@ -268,33 +273,42 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
     for (int i = 0; i < ng; ++i) {
-        if (FPHO_BOXED_OUT(g[i])) continue;
-        const float xc = g[i].xcen, yc = g[i].ycen;
-        const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
-        const float hcs = g[i].hcs, ampA = g[i].ampA;
-        const bool maybe_bright = g[i].peak > tmin;
+        uint32_t m = 0;   // slots with a pixel inside the cutoff (wave-uniform)
+        if (!FPHO_BOXED_OUT(g[i])) {
+            const float xc = g[i].xcen, yc = g[i].ycen;
+            const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
+            const float hcs = g[i].hcs, ampA = g[i].ampA;
+            const bool maybe_bright = g[i].peak > tmin;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float dx = x[k] - xc;
-            const float dy = y[k] - yc;
-            const float vx = sfxx * dx + sfxy * dy;
-            const float vy = sfyy * dy + sfxy * dx;
-            const float ea = dx * vx + dy * vy;      // s E
-            if (!(ea < kCutScaled)) {                 // E <= 36 (cutoff)
-                const float gp = __builtin_amdgcn_exp2f(ea);
-                const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
-                const float c = ampA * gp * h;
-                if (MIXED) {
-                    if (maybe_bright && fabsf(c) > thr[k])
-                        rd[k] -= pair_value64(g64[i], x[k], y[k]);
-                    else
-                        rd[k] -= (double)c;
-                } else {
-                    r[k] -= c;
+            for (int k = 0; k < K; ++k) {
+                const float dx = x[k] - xc;
+                const float dy = y[k] - yc;
+                const float vx = sfxx * dx + sfxy * dy;
+                const float vy = sfyy * dy + sfxy * dx;
+                const float ea = dx * vx + dy * vy;      // s E
+                const bool in = !(ea < kCutScaled);       // E <= 36 (cutoff)
+                if (__builtin_amdgcn_ballot_w64(in) == 0) continue;
+                m |= 1u << k;
+                if (in) {
+                    const float gp = __builtin_amdgcn_exp2f(ea);
+                    const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
+                    const float c = ampA * gp * h;
+                    if (MIXED) {
+                        if (maybe_bright && fabsf(c) > thr[k])
+                            rd[k] -= pair_value64(g64[i], x[k], y[k]);
+                        else
+                            rd[k] -= (double)c;
+                    } else {
+                        r[k] -= c;
+                    }
                 }
             }
         }
+        if (i < mask_cap && lane == 0) amask[i] = (uint8_t)m;
     }
+    // pass 2 of this wave reads the masks written by its lane 0
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 
     // ---- chi2 = sum (r^2 - d^2) ierr^2 in FP64 (compute_gaussians_kernel.cu:276-287)
     double c2 = 0.0;
@@ -328,47 +342,51 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         bool touched = false;
         for (int i = i0; i < i0 + n_psf; ++i) {
             const ImgGauss &G = g[i];
-            if (FPHO_BOXED_OUT(G)) continue;
+            // slots to evaluate: pass 1's mask, else (past the LDS capacity)
+            // every slot of a Gaussian whose box meets the tile
+            uint32_t m;
+            if (i < mask_cap)
+                m = __builtin_amdgcn_readfirstlane((uint32_t)amask[i]);
+            else
+                m = FPHO_BOXED_OUT(G) ? 0u : 0xFFu;
+            if (m == 0) continue;
             touched = true;
             const float xc = G.xcen, yc = G.ycen;
             const float sfxx = G.sfxx, sfyy = G.sfyy, sfxy = G.sfxy, hcs = G.hcs;
             // the 12 per-Gaussian pixel sums (over this lane's K pixels)
             float PA = 0.f, X1 = 0.f, X2 = 0.f, Y1 = 0.f, Y2 = 0.f, Pxx = 0.f;
             float Pyy = 0.f, Pxy = 0.f, Pe = 0.f, Peu = 0.f, Pew = 0.f, Pecr = 0.f;
-            bool any = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
+                if (!(m & (1u << k))) continue;
                 const float dx = x[k] - xc;
                 const float dy = y[k] - yc;
                 const float vx = sfxx * dx + sfxy * dy;
                 const float vy = sfyy * dy + sfxy * dx;
                 const float u = dx * vx;
                 const float ea = fmaf(dy, vy, u);
+                // a pixel beyond the cutoff gets gp = exp2(-inf) = 0, so all
+                // its terms below are exact zeros (every factor is finite)
                 const bool in = !(ea < kCutScaled);
-                if (__builtin_amdgcn_ballot_w64(in) == 0) continue;
-                any = true;
-                if (in) {
-                    const float gp = __builtin_amdgcn_exp2f(ea);
-                    const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
-                    const float ee = rr[k] * gp;
-                    const float a = ee * h;
-                    PA += a;
-                    X1 = fmaf(a, vx, X1);
-                    X2 = fmaf(ee, vx, X2);   // sum e v'x (s F applied below)
-                    Y1 = fmaf(a, vy, Y1);
-                    Y2 = fmaf(ee, vy, Y2);   // sum e v'y
-                    const float ax = a * dx;
-                    const float ay = a * dy;
-                    Pxx = fmaf(ax, dx, Pxx);
-                    Pxy = fmaf(ax, dy, Pxy);
-                    Pyy = fmaf(ay, dy, Pyy);
-                    Pe += ee;
-                    Peu = fmaf(ee, u, Peu);
-                    Pew = fmaf(ee, ea - u, Pew);
-                    Pecr = fmaf(ee, fmaf(dy, vx, dx * vy), Pecr);
-                }
+                const float gp = __builtin_amdgcn_exp2f(in ? ea : -__builtin_huge_valf());
+                const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
+                const float ee = rr[k] * gp;
+                const float a = ee * h;
+                PA += a;
+                X1 = fmaf(a, vx, X1);
+                X2 = fmaf(ee, vx, X2);   // sum e v'x (s F applied below)
+                Y1 = fmaf(a, vy, Y1);
+                Y2 = fmaf(ee, vy, Y2);   // sum e v'y
+                const float ax = a * dx;
+                const float ay = a * dy;
+                Pxx = fmaf(ax, dx, Pxx);
+                Pxy = fmaf(ax, dy, Pxy);
+                Pyy = fmaf(ay, dy, Pyy);
+                Pe += ee;
+                Peu = fmaf(ee, u, Peu);
+                Pew = fmaf(ee, ea - u, Pew);
+                Pecr = fmaf(ee, fmaf(dy, vx, dx * vy), Pecr);
             }
-            if (!any) continue;
             // per-Gaussian epilogue: image-space derivative sums -> 7 partials
             const float gfx = G.k_axx * Pxx + G.k_e * Pe + G.k_u * Peu;
             const float gfy = G.k_axx * Pyy + G.k_e * Pe + G.k_u * Pew;
@@ -424,12 +442,14 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t waves_per_group, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull) {
+    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull, int32_t mask_cap) {
+    extern __shared__ uint8_t lds_mask[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = blockIdx.x * waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull);
+    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull,
+                        lds_mask + w * mask_cap, mask_cap);
 }
 
 // Persistent: every wave pulls tiles from `counter` until the queue is empty.
@@ -439,15 +459,18 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t *__restrict__ counter, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull) {
+    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull, int32_t mask_cap) {
+    extern __shared__ uint8_t lds_mask[];
     const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         int slot = 0;
         if (lane == 0) slot = atomicAdd(counter, 1);
         slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, kWave));
         if (slot >= n_tiles) break;
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull);
+        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull,
+                            lds_mask + w * mask_cap, mask_cap);
     }
 }
 
@@ -694,17 +717,19 @@ hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source 
 template <int K, bool MIXED>
 static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
     const int W = a.waves_per_group;
+    const size_t lds = (size_t)W * (size_t)a.mask_cap;
     if (a.persistent) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
         hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED>), dim3((unsigned)a.grid),
-                           dim3(64 * W), 0, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
+                           dim3(64 * W), lds, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
                            a.counter, a.exps, a.gauss, a.gauss64, a.gpart, a.cpart,
-                           (int32_t)a.cull);
+                           (int32_t)a.cull, (int32_t)a.mask_cap);
     } else {
         const int64_t groups = (a.n_tiles + W - 1) / W;
-        hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), 0,
+        hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), lds,
                            stream, a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps,
-                           a.gauss, a.gauss64, a.gpart, a.cpart, (int32_t)a.cull);
+                           a.gauss, a.gauss64, a.gpart, a.cpart, (int32_t)a.cull,
+                           (int32_t)a.mask_cap);
     }
     return hipGetLastError();
 }

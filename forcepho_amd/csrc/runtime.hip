@@ -139,6 +139,7 @@ struct Plan {
     DevVec<PatchOut> pouts;
     int64_t n_exp = 0, n_tiles = 0, n_outs = 0, n_gauss = 0, n_srcs = 0, n_patches = 0;
     int64_t n_pairs = 0, npix = 0;
+    int32_t max_ng = 0;  // most image Gaussians of one exposure
     std::vector<int64_t> src_off, chi2_off, dchi2_off, theta_off;  // per patch, +1
     fpho_launch launch;
     int precision = kPrecMixed;
@@ -295,6 +296,7 @@ struct HostPlan {
     std::vector<SrcDesc> srcs;
     std::vector<PatchOut> pouts;
     int64_t n_gauss = 0, n_part = 0, n_pairs = 0, npix = 0;
+    int32_t max_ng = 0;   // most image Gaussians of one exposure
 };
 
 // Append one patch to a host plan.  Tiles: each exposure is cut into tiles of
@@ -334,6 +336,7 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
             const int32_t ei = (int32_t)hp.exps.size();
             hp.exps.push_back(ed);
             hp.n_gauss += (int64_t)ns * np;
+            hp.max_ng = std::max(hp.max_ng, ns * np);
             const int32_t n_e = p->exposure_N[e];
             hp.npix += n_e;
             hp.n_pairs += (int64_t)n_e * ns * np;
@@ -428,6 +431,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     plan.n_tiles = (int64_t)hp.tiles.size();
     plan.n_outs = (int64_t)hp.outs.size();
     plan.n_gauss = hp.n_gauss;
+    plan.max_ng = hp.max_ng;
     plan.n_srcs = (int64_t)hp.srcs.size();
     plan.n_patches = (int64_t)patches.size();
     plan.n_pairs = hp.n_pairs;
@@ -451,6 +455,7 @@ EvalArgs eval_args(Plan &plan) {
     a.cpart = plan.cpart.ptr;
     a.pix_per_lane = plan.launch.pix_per_lane;
     a.cull = plan.launch.cull;
+    a.mask_cap = std::min(kMaskCapMax, (plan.max_ng + 3) & ~3);
     a.waves_per_group = plan.launch.waves_per_group;
     a.persistent = plan.launch.persistent;
     const int64_t groups_needed = (plan.n_tiles + a.waves_per_group - 1) / a.waves_per_group;

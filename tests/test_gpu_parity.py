@@ -5,8 +5,9 @@ import pytest
 
 import oracle
 from forcepho_amd import synth
+from forcepho_amd.synth import PatchSpec
 from helpers import (CHI2_RTOL, chi2_relerr, grad_mismatch, make_config_patch,
-                     proposal_for)
+                     oracle_render, proposal_for)
 
 pytestmark = pytest.mark.gpu
 
@@ -92,3 +93,19 @@ def test_config2_culling_bitwise(c2):
     b = patch.get_proposer().evaluate_proposal(prop, patch=patch)
     patch.set_launch(cull=1)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_gaussians_beyond_lds_mask_capacity(hip):
+    """128 sources x 72 image Gaussians = 9216 Gaussians in one exposure, more
+    than the 8192 whose per-slot masks fit in LDS: the Gaussians past that
+    capacity are evaluated on every slot their box allows in pass 2."""
+    spec = PatchSpec(n_sources=128, n_bands=1, n_exp_per_band=1, side=48, n_psf=8)
+    patch, scene, theta = synth.make_patch(spec, config=41, patch_id=0, render=oracle_render)
+    assert patch.n_sources * int(patch.n_psf_per_source[0]) > 8192
+    patch.send_to_device()
+    try:
+        _check(patch, proposal_for(scene, theta), "9216 Gaussians truth")
+        t = synth.make_proposals(theta, 1, 1, config=41)[0]
+        _check(patch, proposal_for(scene, t), "9216 Gaussians proposal")
+    finally:
+        patch.free()
