@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--patches", type=int, default=512)
+    ap.add_argument("--patches", type=int, default=None,
+                    help="patches (default 512 for config 4, 4096 for config 5)")
     ap.add_argument("--plain-sweeps", type=int, default=1)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -52,6 +53,9 @@ def parse():
     ap.add_argument("--tune", type=str, default=None, metavar="CACHE_JSON",
                     help="run the launch autotuner on this workload, write the "
                          "cache to CACHE_JSON and exit")
+    ap.add_argument("--config", type=int, default=4, choices=[4, 5],
+                    help="4: the benchmark line (default); 5: the mosaic-scale stress "
+                         "run (4096 patches, precision ablation, HMC), one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
@@ -100,18 +104,20 @@ def algorithmic_counts(specs):
     return pairs, npix, flops
 
 
-def build_patches(specs, log_every=64):
+def build_patches(specs, log_every=64, config=4, keep_host=None):
     """Upload every patch with data = noise, render the truth on the device
     with negated fluxes (the model is linear in flux) so that the residual
-    buffer holds noise + model, and make it the data (swap_on_device)."""
+    buffer holds noise + model, and make it the data (swap_on_device).
+    Host pixel arrays are dropped after upload except for the patches in
+    ``keep_host`` (None keeps all)."""
     import ctypes
     from forcepho_amd import _lib, synth
     from forcepho_amd.limits import source_struct_dtype
     out = []
     for i, spec in enumerate(specs):
-        patch, scene, theta = synth.make_patch(spec, config=4, patch_id=i,
+        patch, scene, theta = synth.make_patch(spec, config=config, patch_id=i,
                                                return_residual=True)
-        rng = synth.rng_for(4, i, 2)
+        rng = synth.rng_for(config, i, 2)
         noise = np.zeros(patch.npix, dtype=np.float32)
         for e in range(len(patch.exposure_N)):
             b = int(np.searchsorted(np.cumsum(patch.band_N), e, side="right"))
@@ -131,10 +137,35 @@ def build_patches(specs, log_every=64):
                                             _lib.fptr(d, ctypes.c_float)))
         patch.swap_on_device()
         patch.data = None  # host copy no longer describes the device data
+        if keep_host is not None and i not in keep_host:
+            patch.ierr = None   # xpix stays: it sizes the patch (npix)
+            patch.ypix = None
         out.append((patch, scene, theta))
         if (i + 1) % log_every == 0:
             log(f"  built {i + 1}/{len(specs)} patches")
     return out
+
+
+def time_plain(patch_list, props_host, sweeps=1):
+    """Seconds per sweep of synchronous ``HIPProposer.evaluate_proposal``
+    calls, one per patch, as a fitting loop makes them (no residual
+    download: the residual stays on the device)."""
+    from forcepho_amd.proposal import HIPProposer
+    prop_er = HIPProposer()
+    saved = [p.return_residual for p in patch_list]
+    for p in patch_list:
+        p.return_residual = False
+    try:
+        for p, pr in zip(patch_list, props_host):    # warm-up (plans, staging)
+            prop_er.evaluate_proposal(pr, patch=p)
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            for p, pr in zip(patch_list, props_host):
+                prop_er.evaluate_proposal(pr, patch=p)
+        return (time.perf_counter() - t0) / sweeps
+    finally:
+        for p, r in zip(patch_list, saved):
+            p.return_residual = r
 
 
 def cpu_baseline(patches, specs, thetas, seconds):
@@ -177,6 +208,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.config == 5 and args.gpus == 1:
+        run_config5(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
                           "part of it (patches would be sharded across GPUs, see "
@@ -191,7 +225,7 @@ def main():
     _lib.init(0)
     torch.cuda.init()
     t_setup = time.time()
-    specs = synth.config4_specs(args.patches)
+    specs = synth.config4_specs(args.patches or 512)
     pairs, npix, flops = algorithmic_counts(specs)
     log(f"config 4: {len(specs)} patches, {npix:.3e} px, {pairs:.3e} pairs, "
         f"{flops:.3e} algorithmic FLOP per sweep")
@@ -279,19 +313,11 @@ def main():
     # ---- plain single-patch drop-in
     plain = None
     if not args.skip_plain:
-        from forcepho_amd.proposal import HIPProposer
         props_host = []
         for i, (p, sc, th) in enumerate(patches):
             sc.set_all_source_params(thetas[i])
             props_host.append(sc.get_proposal().copy())
-        prop_er = HIPProposer()
-        for (p, sc, th), pr in zip(patches, props_host):    # warm-up (plans, staging)
-            prop_er.evaluate_proposal(pr, patch=p)
-        t0 = time.perf_counter()
-        for _ in range(args.plain_sweeps):
-            for (p, sc, th), pr in zip(patches, props_host):
-                prop_er.evaluate_proposal(pr, patch=p)
-        t_plain = (time.perf_counter() - t0) / args.plain_sweeps
+        t_plain = time_plain([p for p, _, _ in patches], props_host, args.plain_sweeps)
         plain = len(patches) / t_plain
         log(f"plain: {t_plain * 1e3:.1f} ms/sweep, {plain:.1f} evals/s")
 
@@ -335,6 +361,152 @@ def main():
                               "98 per pair inside the cutoff + 11 per other pair + "
                               "7 N_pix n_src + 6 N_pix")},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_config5(args):
+    """Mosaic-scale stress on one GPU: 4096 patches (5% crowded: 60 sources,
+    12 bands x 2 exposures of 256^2) resident in HBM.  For each precision
+    mode: time per sweep, chi2 and gradient error against the FP64 oracle on
+    a sample (the 3 smallest patches and one crowded patch).  Then
+    graph-captured HMC trajectories over all patches, and the plain drop-in
+    timed on a random sample of patches.  Prints one JSON line."""
+    import torch
+    import oracle
+    from forcepho_amd import _lib, synth
+    from forcepho_amd.autotune import load_or_default
+    from forcepho_amd.batch import DeviceSplines, PatchBatch
+    from forcepho_amd.hmc import HMC, DevicePosterior
+    from forcepho_amd.limits import source_struct_dtype
+
+    _lib.init(0)
+    torch.cuda.init()
+    t_setup = time.time()
+    specs = synth.config5_specs(args.patches or 4096)
+    pairs, npix, flops = algorithmic_counts(specs)
+    order = np.argsort([s.n_pairs() for s in specs])
+    crowded = [i for i, s in enumerate(specs) if s.crowded]
+    sample = [int(i) for i in order[:3]] + crowded[:1]
+    log(f"config 5: {len(specs)} patches ({len(crowded)} crowded), {npix:.3e} px, "
+        f"{pairs:.3e} pairs per sweep")
+    patches = build_patches(specs, log_every=512, config=5, keep_host=set(sample))
+    log(f"setup {time.time() - t_setup:.1f}s; device memory "
+        f"{torch.cuda.memory_allocated() / 1e9:.1f} GB (torch) ")
+    free, total = torch.cuda.mem_get_info()
+    resident = (total - free) / 1e9
+    batch = PatchBatch([p for p, _, _ in patches], precision="mixed")
+    launch = load_or_default("batch")
+    batch.set_launch(**launch)
+    splines = DeviceSplines()
+    thetas = [synth.make_proposals(th, len(p.band_N), 1, config=5, patch_id=i)[0]
+              for i, (p, sc, th) in enumerate(patches)]
+    theta = torch.as_tensor(np.concatenate(thetas), dtype=torch.float64, device="cuda")
+
+    # FP64 oracle on the sample (proposals made on the device from theta)
+    batch.theta_to_proposals(theta, splines)
+    props = batch.proposals.cpu().numpy().view(source_struct_dtype)
+    ref = {}
+    for i in sample:
+        p = patches[i][0]
+        p.data = p.retrieve_array("data")
+        t0 = time.time()
+        ref[i] = oracle.evaluate(p, props[batch.src_off[i]:batch.src_off[i + 1]])
+        log(f"oracle patch {i}: {time.time() - t0:.1f}s")
+
+    modes = {}
+    for mode in ("mixed", "fp64_reduce", "fp32"):
+        batch.set_precision(mode)
+        batch.lnlike_and_grad(theta, splines)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.lnlike_and_grad(theta, splines)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        chi2 = batch.chi2_host()
+        dchi2 = batch.dchi2_host()
+        cerr, gbad, gsize, gerr = 0.0, 0, 0, 0.0
+        for i in sample:
+            c_ref, d_ref = ref[i]
+            cerr = max(cerr, float(np.max(np.abs(chi2[i] - c_ref) / np.abs(c_ref))))
+            d = dchi2[i].astype(np.float64)
+            scale = np.abs(d_ref).reshape(d_ref.shape[0], -1).max(axis=1)[:, None, None]
+            err = np.abs(d - d_ref)
+            bad = (err > 1e-4 * np.abs(d_ref)) & (err > 1e-6 * scale)
+            gbad += int(bad.sum())
+            gsize += int(bad.size)
+            gerr = max(gerr, float(np.max(err / scale)))
+        modes[mode] = {"ms_per_sweep": dt * 1e3, "evals_per_s": len(patches) / dt,
+                       "chi2_rel_err_max": cerr, "grad_mismatches": gbad,
+                       "grad_components": gsize, "grad_err_max_over_band_max": gerr}
+        log(f"{mode}: {dt * 1e3:.1f} ms/sweep, chi2 err {cerr:.2e}, grad mismatches "
+            f"{gbad}/{gsize}")
+    batch.set_precision("mixed")
+
+    # algorithmic roofline of the pixel pass (mixed)
+    batch.set_timing(True)
+    batch.lnlike_and_grad(theta, splines)
+    stage = np.array(batch.last_timing())
+    batch.set_timing(False)
+    n_in = batch.count_pairs_in_cutoff()
+    flops_alg = flops - FLOP_PER_PAIR * pairs + FLOP_PER_PAIR * n_in + \
+        FLOP_PER_CUT_PAIR * max(pairs - n_in, 0)
+    achieved = flops_alg / (stage[1] * 1e-3) / 1e12
+
+    # HMC over every patch, one trajectory captured in a graph
+    lo, hi = [], []
+    for (p, sc, th) in patches:
+        l, h = synth.theta_bounds(th, len(p.band_N))
+        lo.append(l)
+        hi.append(h)
+    post = DevicePosterior(batch, splines, np.concatenate(lo), np.concatenate(hi))
+    z0 = post.inverse_transform(np.clip(np.concatenate(thetas), np.concatenate(lo) + 1e-9,
+                                        np.concatenate(hi) - 1e-9))
+    n_steps = 10
+    hmc = HMC(post, z0, step_size=1e-3, n_steps=n_steps, seed=5)
+    hmc.capture(warmup=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_traj = 2
+    hmc.run(n_traj)
+    torch.cuda.synchronize()
+    t_hmc = time.perf_counter() - t0
+    hmc_evals = len(patches) * n_steps * n_traj / t_hmc
+    accept = float(hmc.n_accept.mean().item()) / (n_traj + 1)   # + 1 warm-up
+    log(f"HMC: {n_traj} trajectories x {n_steps} steps in {t_hmc:.2f}s "
+        f"({hmc_evals:.0f} evals/s), mean acceptance {accept:.2f}")
+
+    # plain drop-in on a random sample, scaled per pair to the sweep
+    rng = np.random.default_rng(55)
+    pick = rng.choice(len(patches), min(256, len(patches)), replace=False)
+    props_host = []
+    for i in pick:
+        sc = patches[i][1]
+        sc.set_all_source_params(thetas[i])
+        props_host.append(sc.get_proposal().copy())
+    t_pl = time_plain([patches[i][0] for i in pick], props_host, 1)
+    pairs_pick = sum(specs[i].n_pairs() for i in pick)
+    plain = len(patches) / (t_pl * pairs / pairs_pick)
+    log(f"plain: {len(pick)} patches in {t_pl:.2f}s -> {plain:.1f} evals/s over the sweep")
+
+    out = {
+        "metric": "config 5 mosaic stress: lnlike+grad evals/s per precision mode",
+        "n_gpus": 1, "steps": args.steps, "data": "synthetic",
+        "config": {"workload": f"config 5: {len(patches)} patches, {len(crowded)} crowded "
+                               f"(60 src, 12 bands x 2 exp of 256^2)",
+                   "patches": len(patches), "npix": npix, "n_pairs": pairs,
+                   "launch": launch, "device_resident_gb": resident},
+        "modes": modes,
+        "pairs_in_cutoff": n_in,
+        "roofline_mixed": {"achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": achieved / PEAK_FP32_TFLOPS,
+                           "pixel_pass_ms": float(stage[1])},
+        "hmc": {"trajectories": n_traj, "leapfrog_steps": n_steps, "seconds": t_hmc,
+                "evals_per_s": hmc_evals, "mean_acceptance": accept},
+        "plain_evals_per_s": plain,
+        "batched_over_plain": modes["mixed"]["evals_per_s"] / plain,
+        "oracle_sample": {"patches": sample, "crowded_patch": crowded[:1]},
     }
     print(json.dumps(out), flush=True)
 

@@ -58,6 +58,7 @@ class PatchSpec:
         self.psf_elliptical = bool(psf_elliptical)
         self.psf_sigma = tuple(psf_sigma)
         self.dither = float(dither)
+        self.crowded = False
 
     @property
     def n_exp(self):
@@ -102,6 +103,23 @@ def config4_specs(n_patches=512, config=4, seed_patch0=0):
         npsf = rng.integers(3, 7, size=nb)
         specs.append(PatchSpec(n_sources=ns, n_bands=nb, n_exp_per_band=ne, side=side,
                                n_psf=npsf, superpixel=(8, 8), warp_size=64))
+    return specs
+
+
+def config5_specs(n_patches=4096, crowded_frac=0.05):
+    """Mosaic stress: patches from the configuration-4 distribution, and a
+    crowded tail (a seeded choice of ``crowded_frac`` of them) of 60
+    sources, 12 bands x 2 exposures of 256 x 256 pixels, 3-6 PSF Gaussians
+    per band."""
+    specs = config4_specs(n_patches, config=5)
+    rng = rng_for(5, 0, 4)
+    n_crowd = int(round(crowded_frac * n_patches))
+    for i in sorted(rng.choice(n_patches, n_crowd, replace=False).tolist()):
+        r = rng_for(5, i, 3)
+        specs[i] = PatchSpec(n_sources=60, n_bands=12, n_exp_per_band=2, side=256,
+                             n_psf=r.integers(3, 7, size=12), superpixel=(8, 8),
+                             warp_size=64)
+        specs[i].crowded = True
     return specs
 
 

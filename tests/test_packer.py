@@ -90,3 +90,19 @@ def test_config_specs_sizes():
     specs = synth.config4_specs(16)
     assert all(1 <= s.n_sources <= 30 and 1 <= s.n_bands <= 10 for s in specs)
     assert all(32 <= s.side <= 256 and np.all(s.n_exp_per_band <= 16) for s in specs)
+
+
+def test_config5_specs_crowded_tail():
+    specs = synth.config5_specs(4096)
+    crowded = [s for s in specs if s.crowded]
+    assert len(specs) == 4096 and len(crowded) == 205
+    for s in crowded:
+        assert s.n_sources == 60 and s.n_bands == 12 and s.n_exp == 24 and s.side == 256
+    # the rest follow the configuration-4 distribution
+    rest = [s for s in specs if not s.crowded]
+    assert max(s.n_sources for s in rest) <= 30 and max(s.n_bands for s in rest) <= 10
+    npix = sum(s.npix_unpadded for s in specs)
+    assert 4.5e9 < npix < 5.5e9   # about 39 GB of data + ierr (f32)
+    # seeded: the same call gives the same patches
+    again = synth.config5_specs(4096)
+    assert [s.n_pairs() for s in again] == [s.n_pairs() for s in specs]

@@ -1,0 +1,43 @@
+"""The rocprofv3 summary helpers on a small database of the same schema."""
+import sqlite3
+
+import pytest
+
+from forcepho_amd import profiling
+
+
+@pytest.fixture()
+def prof_dir(tmp_path):
+    d = tmp_path / "run"
+    d.mkdir()
+    con = sqlite3.connect(str(d / "run_results.db"))
+    con.execute("create table counters_collection (kernel_name text, dispatch_id int, "
+                "counter_name text, value real, duration int)")
+    rows = [("void fpho::eval_kernel<8, true>(args)", k, "FETCH_SIZE", 100.0 + k, 115_000_000)
+            for k in range(5)]
+    rows += [("void fpho::eval_kernel<4, true>(args)", 10 + k, "FETCH_SIZE", 1.0, 500_000)
+             for k in range(3)]
+    con.executemany("insert into counters_collection values (?, ?, ?, ?, ?)", rows)
+    con.execute("create table top_kernels (name text, total_calls int, total_duration real, "
+                "average real, percentage real)")
+    con.execute("insert into top_kernels values ('k', 2, 10.0, 5.0, 100.0)")
+    con.commit()
+    con.close()
+    return str(d)
+
+
+def test_counter_medians_filters_short_dispatches(prof_dir):
+    med = profiling.counter_medians(prof_dir, min_ms=10)
+    assert list(med) == [("void fpho::eval_kernel<8, true>", "FETCH_SIZE")]
+    n, value, dur = med[("void fpho::eval_kernel<8, true>", "FETCH_SIZE")]
+    assert n == 5 and value == 102.0 and dur == 115_000_000
+    assert len(profiling.counter_medians(prof_dir)) == 2
+
+
+def test_kernel_stats_and_summary_round_trip(prof_dir, tmp_path):
+    assert profiling.kernel_stats(prof_dir) == [("k", 2, 10.0, 5.0, 100.0)]
+    f = tmp_path / "s.txt"
+    f.write_text("# comment: x\nlaunch: 8,1,0,8\ntraffic_bytes_per_launch: 1.5e+10\n")
+    kv = profiling.read_summary(str(f))
+    assert kv == {"launch": "8,1,0,8", "traffic_bytes_per_launch": "1.5e+10"}
+    assert len(profiling.kernel_source_digest()) == 16
