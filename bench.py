@@ -356,10 +356,9 @@ def main():
                      "algorithmic_flop_per_sweep": flops_alg,
                      "pairs_in_cutoff": n_in,
                      "note": ("compute-bound on the FP32 VALU (no dense contraction, so no "
-                              "MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s, of which "
-                              "78.6 is issuable without packed FP32; algorithmic FLOP = "
-                              "98 per pair inside the cutoff + 11 per other pair + "
-                              "7 N_pix n_src + 6 N_pix")},
+                              "MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s (the build "
+                              "uses no packed FP32); algorithmic FLOP = 98 per pair inside "
+                              "the cutoff + 11 per other pair + 7 N_pix n_src + 6 N_pix")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -131,10 +131,12 @@ struct alignas(8) PatchOut {
     int32_t pad;
 };
 
-// FP64 copy of an image Gaussian's value parameters, used to re-evaluate the
-// few (pixel, Gaussian) terms that are bright relative to the pixel noise.
+// FP64 copy of an image Gaussian's value parameters in the scaled variables
+// of ImgGauss (sF, hcs = s^2 (24 - tr F), ampA = amp / (24 s^2)), used to
+// re-evaluate the few (pixel, Gaussian) terms that are bright relative to
+// the pixel noise.
 struct alignas(16) ImgGauss64 {
-    double xcen, ycen, fxx, fyy, fxy, amp, pad0, pad1;
+    double xcen, ycen, sfxx, sfyy, sfxy, hcs, ampA, pad0;
 };
 static_assert(sizeof(ImgGauss64) == 64, "ImgGauss64 must be 64 bytes");
 

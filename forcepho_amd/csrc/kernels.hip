@@ -132,12 +132,12 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
 
     g64.xcen = xc;
     g64.ycen = yc;
-    g64.fxx = F.a;
-    g64.fyy = F.d;
-    g64.fxy = F.c;
-    g64.amp = amp;
+    g64.sfxx = sc * F.a;
+    g64.sfyy = sc * F.d;
+    g64.sfxy = sc * F.c;
+    g64.hcs = sc2 * (24.0 - F.a - F.d);
+    g64.ampA = amp / (24.0 * sc2);
     g64.pad0 = 0.0;
-    g64.pad1 = 0.0;
 }
 
 // One workgroup per exposure; lanes stride over its n_src * n_psf Gaussians.
@@ -180,22 +180,34 @@ __device__ __forceinline__ bool boxed_out(int cull, const int4 gb, const int4 tb
     return cull != 0 && miss != 0;
 }
 
-// Value of one image Gaussian at one pixel in FP64 (bright terms only).
+// Value of one image Gaussian at one pixel in FP64 (bright terms only), in
+// the scaled variables of ImgGauss: ampA 2^E' (hcs + v'.v').  2^E' for
+// E' in [36 s, 0] is 2^n 2^f with n = rint(E'), |f| <= 1/2, and 2^f =
+// (e^y)^1024 with y = f ln2 / 1024, |y| < 3.4e-4, e^y by its cubic Taylor
+// polynomial (truncation 6e-16, amplified 1024-fold by the squarings to
+// < 1e-12 relative) -- two constants, so the rare FP64 path holds few
+// registers.
 __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, float yp) {
     const double dx = (double)xp - g.xcen;
     const double dy = (double)yp - g.ycen;
-    const double vx = g.fxx * dx + g.fxy * dy;
-    const double vy = g.fyy * dy + g.fxy * dx;
+    const double vx = g.sfxx * dx + g.sfxy * dy;
+    const double vy = g.sfyy * dy + g.sfxy * dx;
     const double ea = dx * vx + dy * vy;
-    const double h = 1.0 + (vx * vx + vy * vy - g.fxx - g.fyy) * (1.0 / 24.0);
-    return g.amp * exp(-0.5 * ea) * h;
+    const double n = __builtin_rint(ea);
+    const double y = (ea - n) * (0.69314718055994530942 / 1024.0);
+    double p = fma(y, fma(y, fma(y, 1.0 / 6.0, 0.5), 1.0), 1.0);
+#pragma unroll
+    for (int j = 0; j < 10; ++j) p *= p;
+    const double h = fma(vx, vx, fma(vy, vy, g.hcs));
+    return g.ampA * __builtin_ldexp(p, (int)n) * h;
 }
 
 // Evaluate one wave tile.  `tile` must be wave-uniform (an SGPR value) so
 // every descriptor and Gaussian load below is a scalar load.
 //   MIXED = true : residual accumulated in FP64; a term whose FP32 value
-//                  exceeds kBrightSigma * sigma of its pixel is re-evaluated
-//                  in FP64 (a wave branch taken only near bright cores).
+//                  exceeds kBrightSigma * sigma of the tile's quietest pixel
+//                  is re-evaluated in FP64 (a wave branch taken only near
+//                  bright cores; about 3% of slot evaluations on config 4).
 //   MIXED = false: FP32 residual, as forcepho/src/common_kernel.cc:57-83.
 //
 // Scaled variables (ImgGauss): d = pixel - centre, v' = s F d, E' = d.v' =
@@ -233,26 +245,26 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     const int n_psf = e.n_psf;
     const int ng = n_src * n_psf;
 
-    float x[K], y[K], d[K], ie[K], r[K], thr[K];
+    float x[K], y[K], r[K];
     double rd[K];
+    float tmin = __builtin_huge_valf();   // smallest bright threshold in the tile
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = k * kWave + lane;
+        float dk = 0.0f;
         if (p < t.npix) {
             const int i = t.pix0 + p;
             x[k] = e.xpix[i];
             y[k] = e.ypix[i];
-            d[k] = e.data[i];
-            ie[k] = e.ierr[i];
+            dk = e.data[i];
+            const float iek = e.ierr[i];
+            if (MIXED && iek > 0.0f) tmin = fminf(tmin, kBrightSigma / iek);
         } else {
             x[k] = kMaskedCoord;
             y[k] = kMaskedCoord;
-            d[k] = 0.0f;
-            ie[k] = 0.0f;
         }
-        r[k] = d[k];
-        rd[k] = (double)d[k];
-        thr[k] = ie[k] > 0.0f ? kBrightSigma / ie[k] : __builtin_huge_valf();
+        r[k] = dk;
+        rd[k] = (double)dk;
     }
 
     // A Gaussian whose cutoff box misses the tile's pixel box contributes
@@ -260,15 +272,13 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
 #define FPHO_BOXED_OUT(G) boxed_out(cull, *reinterpret_cast<const int4 *>(&(G).xlo), tbox)
 
-    // smallest bright-term threshold over the tile: a Gaussian whose peak
-    // stays below it cannot have a bright term here
-    float tmin = thr[0];
-#pragma unroll
-    for (int k = 1; k < K; ++k) tmin = fminf(tmin, thr[k]);
+    // A term counts as bright above 0.1 sigma of the tile's quietest pixel;
+    // a Gaussian whose peak stays below that cannot have a bright term here.
     if (MIXED) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, off, kWave));
-        tmin = __builtin_amdgcn_readfirstlane(tmin);
+        // readfirstlane is an integer builtin: pass the float's bits
+        tmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tmin)));
     }
 
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
@@ -294,7 +304,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                     const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
                     const float c = ampA * gp * h;
                     if (MIXED) {
-                        if (maybe_bright && fabsf(c) > thr[k])
+                        if (maybe_bright && fabsf(c) > tmin)
                             rd[k] -= pair_value64(g64[i], x[k], y[k]);
                         else
                             rd[k] -= (double)c;
@@ -316,20 +326,25 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = k * kWave + lane;
+        float dk = 0.0f, iek = 0.0f;
+        if (p < t.npix) {
+            dk = e.data[t.pix0 + p];
+            iek = e.ierr[t.pix0 + p];
+        }
         if (MIXED) {
             r[k] = (float)rd[k];
-            const double rs = rd[k] * (double)ie[k];
-            const double ds = (double)d[k] * (double)ie[k];
+            const double rs = rd[k] * (double)iek;
+            const double ds = (double)dk * (double)iek;
             c2 += (rs - ds) * (rs + ds);
         } else {
-            const float rs = r[k] * ie[k];
-            const float ds = d[k] * ie[k];
+            const float rs = r[k] * iek;
+            const float ds = dk * iek;
             double c = (double)rs - (double)ds;
             c *= (double)rs + (double)ds;
             c2 += c;
         }
         if (e.residual != nullptr && p < t.npix) e.residual[t.pix0 + p] = r[k];
-        rr[k] = -2.0f * ((r[k] * ie[k]) * ie[k]);  // -2 r ierr^2
+        rr[k] = -2.0f * ((r[k] * iek) * iek);  // -2 r ierr^2
     }
     c2 = wave_sum(c2);
     if (lane == 0) cpart[tile] = c2;
@@ -436,6 +451,23 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 #undef FPHO_BOXED_OUT
 }
 
+// Workgroup b runs on XCD b % 8 (round-robin dispatch); map it to queue
+// group (chunk-row, XCD, offset) so that each XCD takes chunks of C
+// consecutive groups (tiles of one exposure share image Gaussians in its L2
+// and scalar caches) and the chunks interleave across XCDs (the cost-sorted
+// queue stays balanced).  A bijection on [0, n); the tail past the last full
+// row of 8*C groups maps to itself.
+constexpr int kXcdChunk = 64;   // queue groups per XCD chunk (measured: 16, 64, 256)
+
+template <int C>
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+    constexpr int X = 8, span = X * C;
+    const int full = (n / span) * span;
+    if (b >= full) return b;
+    const int xcd = b % X, local = b / X;
+    return (local / C) * span + xcd * C + (local % C);
+}
+
 // Non-persistent: wave w of workgroup b evaluates queue[b*W + w].
 template <int K, bool MIXED>
 __global__ __launch_bounds__(256) void eval_kernel(
@@ -445,7 +477,8 @@ __global__ __launch_bounds__(256) void eval_kernel(
     float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull, int32_t mask_cap) {
     extern __shared__ uint8_t lds_mask[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int slot = blockIdx.x * waves_per_group + w;
+    const int slot = xcd_swizzle<kXcdChunk>((int)blockIdx.x, (int)gridDim.x) *
+                     waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
     eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull,
