@@ -41,12 +41,15 @@ class DevicePatchMixin:
                                   **model_kwargs)
         return model, q
 
-    def subtract_fixed(self, fixed, active=None, maxactive=15, swap_local=False):
-        """Subtract blocks of fixed sources from the device-side data
+    def subtract_fixed(self, fixed, active=None, big=None, maxactive=15, swap_local=False):
+        """Subtract blocks of big and fixed sources from the device-side data
         (forcepho/patches/device_patch.py:113-183).  After the call the
-        device data hold the residual of all fixed sources and the device
-        metadata describe the last scene (``active`` if given)."""
+        device data hold the residual of all subtracted sources and the
+        device metadata describe the last scene (``active`` if given)."""
         scenes = []
+        if big is not None:
+            inds = np.arange(maxactive, len(big), maxactive)
+            scenes.extend([self.set_scene(block) for block in np.array_split(big, inds)])
         if fixed is not None:
             inds = np.arange(maxactive, len(fixed), maxactive)
             scenes.extend([self.set_scene(block) for block in np.array_split(fixed, inds)])

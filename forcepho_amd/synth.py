@@ -206,6 +206,7 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
     patch.pack_pix(images, ierrs, bands, superpixel=spec.superpixel,
                    warp_size=spec.warp_size)
     scene = Scene(catalog=cat, filternames=bandlist)
+    patch.truth_catalog = cat
     patch.pack_meta(scene)
     theta = scene.get_all_source_params().copy()
     patch.sigma_band = sigmas
