@@ -87,6 +87,10 @@ struct alignas(16) ExpDesc {
     int32_t band;                            // band index within the patch
     int32_t n_src, n_psf, n_radii;
     float G;
+    // pixel origin of the exposure: the pass works in coordinates relative
+    // to it (exact for pixel coordinates), so FP32 centres keep their
+    // precision however far the patch sits from the frame's origin
+    float x0, y0;
 };
 static_assert(sizeof(ExpDesc) == 128, "ExpDesc must be 128 bytes");
 

@@ -72,8 +72,8 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
     const double am = src.mixture_amplitudes[bin];
     const double tmp = (double)e.G * (double)pg.amp * sqrt(detF) * (0.5 / 3.14159265358979323846);
     const double amp = tmp * flux * am;
-    const double xc = xs + (double)pg.xcen;
-    const double yc = ys + (double)pg.ycen;
+    const double xc = xs + (double)pg.xcen - (double)e.x0;   // relative to the origin
+    const double yc = ys + (double)pg.ycen - (double)e.y0;
 
     // dSigma = r2 (T dT^T + dT T^T); dF = -F dSigma F
     const M dSq = scale(add(mul(T, transpose(dT_dq)), mul(dT_dq, Tt)), r2);
@@ -254,8 +254,8 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         float dk = 0.0f;
         if (p < t.npix) {
             const int i = t.pix0 + p;
-            x[k] = e.xpix[i];
-            y[k] = e.ypix[i];
+            x[k] = e.xpix[i] - e.x0;
+            y[k] = e.ypix[i] - e.y0;
             dk = e.data[i];
             const float iek = e.ierr[i];
             if (MIXED && iek > 0.0f) tmin = fminf(tmin, kBrightSigma / iek);
@@ -527,8 +527,8 @@ __global__ __launch_bounds__(256) void count_pairs_kernel(
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = k * kWave + lane;
-        x[k] = p < t.npix ? e.xpix[t.pix0 + p] : kMaskedCoord;
-        y[k] = p < t.npix ? e.ypix[t.pix0 + p] : kMaskedCoord;
+        x[k] = p < t.npix ? e.xpix[t.pix0 + p] - e.x0 : kMaskedCoord;
+        y[k] = p < t.npix ? e.ypix[t.pix0 + p] - e.y0 : kMaskedCoord;
     }
     const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
     unsigned long long n = 0;

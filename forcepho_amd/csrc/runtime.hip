@@ -164,7 +164,8 @@ struct fpho_patch {
     // coordinate box of every 64-pixel chunk of every exposure (chunks start
     // at the exposure start): xlo, xhi, ylo, yhi as order keys
     std::vector<int64_t> chunk_start;  // [n_exp+1] first chunk of exposure e
-    std::vector<int32_t> chunk_box;    // [n_chunks][4]
+    std::vector<int32_t> chunk_box;    // [n_chunks][4], coordinates relative to (x0, y0)
+    std::vector<float> x0, y0;         // [n_exp] pixel origin of each exposure
     DevVec<float> data, ierr, xpix, ypix, residual;
     DevVec<float> rad2, D, crpix, crval, CW;
     DevVec<fpho_psf_gauss> psf;
@@ -333,6 +334,8 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
             ed.n_psf = np;
             ed.n_radii = p->n_radii;
             ed.G = p->G_host[e];
+            ed.x0 = p->x0[e];
+            ed.y0 = p->y0[e];
             const int32_t ei = (int32_t)hp.exps.size();
             hp.exps.push_back(ed);
             hp.n_gauss += (int64_t)ns * np;
@@ -551,13 +554,18 @@ int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch 
     p->chunk_start.assign(1, 0);
     for (int ex = 0; ex < d->n_exp; ++ex) {
         const int64_t s0 = d->exposure_start[ex], n = d->exposure_N[ex];
+        // origin: the exposure's first pixel (a real pixel: padding comes last)
+        const float ox = n > 0 ? std::floor(d->xpix[s0]) : 0.0f;
+        const float oy = n > 0 ? std::floor(d->ypix[s0]) : 0.0f;
+        p->x0.push_back(ox);
+        p->y0.push_back(oy);
         for (int64_t c = 0; c < n; c += kWave) {
-            float xlo = d->xpix[s0 + c], xhi = xlo, ylo = d->ypix[s0 + c], yhi = ylo;
+            float xlo = d->xpix[s0 + c] - ox, xhi = xlo, ylo = d->ypix[s0 + c] - oy, yhi = ylo;
             for (int64_t i = s0 + c; i < s0 + std::min<int64_t>(n, c + kWave); ++i) {
-                xlo = std::min(xlo, d->xpix[i]);
-                xhi = std::max(xhi, d->xpix[i]);
-                ylo = std::min(ylo, d->ypix[i]);
-                yhi = std::max(yhi, d->ypix[i]);
+                xlo = std::min(xlo, d->xpix[i] - ox);
+                xhi = std::max(xhi, d->xpix[i] - ox);
+                ylo = std::min(ylo, d->ypix[i] - oy);
+                yhi = std::max(yhi, d->ypix[i] - oy);
             }
             p->chunk_box.push_back(order_key(xlo));
             p->chunk_box.push_back(order_key(xhi));

@@ -44,7 +44,7 @@ class PatchSpec:
 
     def __init__(self, n_sources, n_bands, n_exp_per_band, side, n_psf,
                  superpixel=(8, 8), warp_size=64, mask_frac=0.0, psf_per_exposure=False,
-                 psf_elliptical=False, psf_sigma=(0.5, 3.0), dither=0.0):
+                 psf_elliptical=False, psf_sigma=(0.5, 3.0), dither=0.0, pix_offset=0.0):
         self.n_sources = int(n_sources)
         self.n_bands = int(n_bands)
         self.n_exp_per_band = np.broadcast_to(np.asarray(n_exp_per_band, dtype=int),
@@ -58,6 +58,7 @@ class PatchSpec:
         self.psf_elliptical = bool(psf_elliptical)
         self.psf_sigma = tuple(psf_sigma)
         self.dither = float(dither)
+        self.pix_offset = float(pix_offset)   # added to every pixel coordinate
         self.crowded = False
 
     @property
@@ -184,7 +185,7 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
             D = Rm / SCALE
             cd = np.linalg.inv(D) / 3600.0
             dith = rng.uniform(-spec.dither, spec.dither, size=2) if spec.dither else np.zeros(2)
-            crpix = np.array([(side - 1) / 2.0, (side - 1) / 2.0]) + dith
+            crpix = np.array([(side - 1) / 2.0, (side - 1) / 2.0]) + dith + spec.pix_offset
             wcses.append(AffineWCS(crpix, (ra0, dec0), cd))
             if spec.psf_per_exposure or b not in band_psf:
                 band_psf[b] = _psf_rows(rng, int(spec.n_psf[b]), spec.psf_sigma,
@@ -205,6 +206,9 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
     patch.psfs = psfs
     patch.pack_pix(images, ierrs, bands, superpixel=spec.superpixel,
                    warp_size=spec.warp_size)
+    if spec.pix_offset:
+        patch.xpix = (patch.xpix + np.float32(spec.pix_offset)).astype(np.float32)
+        patch.ypix = (patch.ypix + np.float32(spec.pix_offset)).astype(np.float32)
     scene = Scene(catalog=cat, filternames=bandlist)
     patch.truth_catalog = cat
     patch.pack_meta(scene)

@@ -109,3 +109,26 @@ def test_gaussians_beyond_lds_mask_capacity(hip):
         _check(patch, proposal_for(scene, t), "9216 Gaussians proposal")
     finally:
         patch.free()
+
+
+def test_large_pixel_coordinates(hip):
+    """Configuration 2's geometry with every pixel coordinate (and the WCS
+    reference pixel) moved by +4000, as in a patch cut from a large detector
+    frame: the parity must not depend on where the patch sits."""
+    base = synth.config_spec(2)
+    spec = PatchSpec(n_sources=base.n_sources, n_bands=base.n_bands,
+                     n_exp_per_band=base.n_exp_per_band, side=base.side, n_psf=base.n_psf,
+                     superpixel=base.superpixel, warp_size=base.warp_size,
+                     mask_frac=base.mask_frac,
+                     psf_per_exposure=base.psf_per_exposure,
+                     psf_elliptical=base.psf_elliptical, psf_sigma=base.psf_sigma,
+                     dither=base.dither, pix_offset=4000.0)
+    patch, scene, theta = synth.make_patch(spec, config=44, patch_id=0, render=oracle_render)
+    assert patch.xpix.max() > 4000
+    patch.send_to_device()
+    try:
+        _check(patch, proposal_for(scene, theta), "offset 4000 truth")
+        t = synth.make_proposals(theta, base.n_bands, 1, config=44)[0]
+        _check(patch, proposal_for(scene, t), "offset 4000 proposal")
+    finally:
+        patch.free()
