@@ -7,11 +7,16 @@ gradient from the HIP kernels -> lnp = lnlike + ln|dq/dz|,
 grad_z = grad_q dq/dz + (1 - 2 sigmoid(z)).  Patches are independent, so
 lnp is returned per patch.
 
-``HMC`` runs fixed-length trajectories (identity mass matrix, per-patch
-Metropolis acceptance) entirely on the device; ``capture()`` records one
-trajectory in a ``torch.cuda.CUDAGraph`` so that a sampler replays it with
-one launch.  This is the loop that littlemcmc drives from Python in the
-reference (forcepho/fitting.py:255-342).
+``HMC`` runs fixed-length trajectories (diagonal mass matrix, a step size
+per patch, per-patch Metropolis acceptance) entirely on the device;
+``capture()`` records one trajectory in a ``torch.cuda.CUDAGraph`` so that a
+sampler replays it with one launch.  ``adapt()`` is the warm-up the
+reference gets from littlemcmc's NUTS (forcepho/fitting.py:255-342: step
+size tuned to a target acceptance, diagonal mass matrix adapted from the
+warm-up draws): Nesterov dual averaging of each patch's step size (Hoffman &
+Gelman 2014, Algorithm 5) and a regularised diagonal inverse mass matrix from
+the draws of a middle window, after which the step size adaptation restarts.
+Patches are independent chains, so every patch adapts its own step size.
 """
 
 import numpy as np
@@ -69,29 +74,46 @@ class DevicePosterior:
 
 
 class HMC:
-    """Fixed-length HMC with identity mass and per-patch acceptance."""
+    """Fixed-length HMC with a diagonal mass matrix, a step size per patch and
+    per-patch acceptance."""
 
-    def __init__(self, posterior, z0, step_size=1e-3, n_steps=10, seed=0):
+    def __init__(self, posterior, z0, step_size=1e-3, n_steps=10, seed=0, inv_mass=None,
+                 jitter=0.2):
         self.post = posterior
+        # each trajectory draws its step sizes uniformly in eps (1 +- jitter),
+        # so a fixed-length trajectory cannot lock onto a period of the target
+        self.jitter = float(jitter)
         dev = posterior.lower.device
-        self.eps = float(step_size)
         self.n_steps = int(n_steps)
         self.z = torch.as_tensor(z0, dtype=torch.float64, device=dev).clone()
         lnp, grad = posterior(self.z)
         self.lnp = lnp.clone()
         self.grad = grad.clone()
         self.n_accept = torch.zeros_like(self.lnp)
+        self.accept_prob = torch.zeros_like(self.lnp)   # of the last trajectory
+        self.eps_patch = torch.full_like(self.lnp, float(step_size))
+        self.inv_mass = (torch.ones_like(self.z) if inv_mass is None else
+                         torch.as_tensor(inv_mass, dtype=torch.float64, device=dev).clone())
         # the default device generator is the one torch graphs replay safely
-        torch.cuda.manual_seed(int(seed))
+        if dev.type == "cuda":
+            torch.cuda.manual_seed(int(seed))
+        else:
+            torch.manual_seed(int(seed))
         self.graph = None
 
-    def leapfrog(self, z, p, grad):
+    @property
+    def eps(self):
+        """Step size of every parameter (its patch's)."""
+        return self.eps_patch[self.post.param_patch]
+
+    def leapfrog(self, z, p, grad, eps=None):
         """n_steps leapfrog steps from (z, p) with gradient ``grad`` at z."""
-        eps = self.eps
+        if eps is None:
+            eps = self.eps
         p = p + 0.5 * eps * grad
         lnp = None
         for i in range(self.n_steps):
-            z = z + eps * p
+            z = z + eps * self.inv_mass * p
             lnp, grad = self.post(z)
             p = p + (eps if i < self.n_steps - 1 else 0.5 * eps) * grad
         return z, p, lnp, grad
@@ -101,16 +123,85 @@ class HMC:
         tensors in place (graph-capturable)."""
         post = self.post
         p0 = torch.randn(self.z.shape, dtype=torch.float64, device=self.z.device)
-        h0 = -self.lnp + 0.5 * post.segment_sum(p0 * p0)
-        z1, p1, lnp1, grad1 = self.leapfrog(self.z, p0, self.grad)
-        h1 = -lnp1 + 0.5 * post.segment_sum(p1 * p1)
+        p0 = p0 * torch.rsqrt(self.inv_mass)
+        h0 = -self.lnp + 0.5 * post.segment_sum(p0 * p0 * self.inv_mass)
+        eps = self.eps_patch
+        if self.jitter > 0.0:
+            u = torch.rand(eps.shape, dtype=torch.float64, device=eps.device)
+            eps = eps * (1.0 + self.jitter * (2.0 * u - 1.0))
+        z1, p1, lnp1, grad1 = self.leapfrog(self.z, p0, self.grad, eps[post.param_patch])
+        h1 = -lnp1 + 0.5 * post.segment_sum(p1 * p1 * self.inv_mass)
+        dh = h0 - h1
+        # a diverging trajectory (non-finite energy) is rejected
+        dh = torch.where(torch.isfinite(dh), dh, torch.full_like(dh, -torch.inf))
+        self.accept_prob.copy_(torch.exp(torch.clamp(dh, max=0.0)))
         u = torch.rand(self.lnp.shape, dtype=torch.float64, device=self.z.device)
-        acc = torch.log(u) < (h0 - h1)
+        acc = torch.log(u) < dh
         acc_p = acc[post.param_patch]
         self.z.copy_(torch.where(acc_p, z1, self.z))
         self.grad.copy_(torch.where(acc_p, grad1, self.grad))
         self.lnp.copy_(torch.where(acc, lnp1, self.lnp))
         self.n_accept.add_(acc.to(torch.float64))
+
+    def adapt(self, n_warmup, target_accept=0.8, gamma=0.05, t0=10.0, kappa=0.75,
+              fast=(0.15, 0.1), first_window=25):
+        """Warm-up in Stan's shape: dual-averaging step sizes (per patch)
+        toward ``target_accept`` throughout; between an initial and a final
+        fast phase (fractions ``fast`` of the warm-up), windows of doubling
+        length (from ``first_window``) each end by setting the inverse mass
+        matrix to the per-parameter variance of the window's draws
+        (shrunk toward 1e-3 of the patch's mean variance) and restarting the
+        step size adaptation.  Ends with each patch's averaged step size.
+        Runs eagerly; capture afterwards."""
+        n_warmup = int(n_warmup)
+        start = int(fast[0] * n_warmup)
+        stop = n_warmup - int(fast[1] * n_warmup)
+        ends, w, b = set(), int(first_window), start
+        while b + w < stop:
+            # a window that would leave less than twice its length is stretched
+            if b + 3 * w > stop:
+                break
+            ends.add(b + w - 1)
+            b += w
+            w *= 2
+        if stop > b:
+            ends.add(stop - 1)
+
+        def restart():
+            mu = torch.log(10.0 * self.eps_patch)
+            return mu, torch.zeros_like(mu), torch.zeros_like(mu), 0
+
+        mu, hbar, log_eps_bar, m = restart()
+        mean = torch.zeros_like(self.z)
+        m2 = torch.zeros_like(self.z)
+        n_mass = 0
+        for it in range(n_warmup):
+            self.trajectory()
+            m += 1
+            a = self.accept_prob
+            hbar = (1.0 - 1.0 / (m + t0)) * hbar + (target_accept - a) / (m + t0)
+            log_eps = mu - (m ** 0.5) / gamma * hbar
+            eta = m ** (-kappa)
+            log_eps_bar = eta * log_eps + (1.0 - eta) * log_eps_bar
+            self.eps_patch.copy_(torch.exp(torch.clamp(log_eps, -20.0, 3.0)))
+            if start <= it < stop:
+                n_mass += 1
+                delta = self.z - mean
+                mean += delta / n_mass
+                m2 += delta * (self.z - mean)
+                if it in ends and n_mass > 2:
+                    var = m2 / (n_mass - 1)
+                    scale = self.post.segment_sum(var) / self.post.segment_sum(
+                        torch.ones_like(var))
+                    self.inv_mass.copy_((n_mass / (n_mass + 5.0)) * var + 1e-3 *
+                                        (5.0 / (n_mass + 5.0)) * scale[self.post.param_patch])
+                    mu, hbar, log_eps_bar, m = restart()
+                    mean.zero_()
+                    m2.zero_()
+                    n_mass = 0
+        if m > 0:
+            self.eps_patch.copy_(torch.exp(torch.clamp(log_eps_bar, -20.0, 3.0)))
+        self.n_accept.zero_()
 
     def capture(self, warmup=2):
         """Record one trajectory in a CUDA/HIP graph (after eager warm-up on

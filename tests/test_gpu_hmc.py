@@ -60,3 +60,20 @@ def test_hmc_1000_leapfrog_steps_in_graph(post):
     # the state stays near the truth in the constrained space
     q = p.transform(hmc.z).cpu().numpy()
     assert np.all(np.isfinite(q))
+
+
+def test_adapt_then_capture(post):
+    """Warm-up on the device posterior, then graph-captured sampling with the
+    adapted per-patch step size and diagonal mass matrix."""
+    p, theta = post
+    z0 = p.inverse_transform(theta)
+    hmc = HMC(p, z0, step_size=1e-5, n_steps=10, seed=2)
+    hmc.adapt(150, target_accept=0.8)
+    assert torch.isfinite(hmc.eps_patch).all() and (hmc.eps_patch > 1e-5).all()
+    assert torch.isfinite(hmc.inv_mass).all() and (hmc.inv_mass > 0).all()
+    hmc.capture(warmup=2)
+    hmc.run(60)
+    torch.cuda.synchronize()
+    acc = (hmc.n_accept / 62.0).cpu().numpy()
+    assert np.all((acc > 0.3) & (acc <= 1.0)), acc
+    assert torch.isfinite(hmc.z).all()

@@ -1,0 +1,61 @@
+"""HMC warm-up (step size dual averaging, diagonal mass matrix) on an
+analytic target, on the CPU: independent Gaussian 'patches' with very
+different scales per parameter."""
+import numpy as np
+import torch
+
+from forcepho_amd.hmc import HMC, segment_sum_index
+
+
+class GaussianPatches:
+    """lnp per patch = -1/2 sum_j (z_j / sigma_j)^2 over the patch's
+    parameters; the attributes HMC reads from a DevicePosterior."""
+
+    def __init__(self, sigmas, theta_off):
+        self.sigma = torch.as_tensor(sigmas, dtype=torch.float64)
+        self.lower = torch.zeros_like(self.sigma)
+        idx, mask = segment_sum_index(theta_off)
+        self.seg_idx = torch.as_tensor(idx)
+        self.seg_mask = torch.as_tensor(mask, dtype=torch.float64)
+        counts = np.diff(theta_off)
+        self.param_patch = torch.as_tensor(np.repeat(np.arange(len(counts)), counts))
+
+    def segment_sum(self, v):
+        return (v[self.seg_idx] * self.seg_mask).sum(dim=1)
+
+    def __call__(self, z):
+        return (-0.5 * self.segment_sum((z / self.sigma) ** 2), -z / self.sigma ** 2)
+
+
+def test_adapt_reaches_target_acceptance_and_mass():
+    rng = np.random.default_rng(0)
+    theta_off = np.array([0, 3, 7, 9])              # three patches, uneven sizes
+    sigmas = 10.0 ** rng.uniform(-2, 0, size=9)     # scales over 2 decades
+    post = GaussianPatches(sigmas, theta_off)
+    z0 = torch.as_tensor(rng.normal(size=9) * sigmas * 0.1)
+    hmc = HMC(post, z0, step_size=1e-4, n_steps=8, seed=3)
+    hmc.adapt(600, target_accept=0.8)
+    # the inverse mass approaches each parameter's variance
+    ratio = (hmc.inv_mass / post.sigma ** 2).numpy()
+    assert np.all((ratio > 0.25) & (ratio < 4.0)), ratio
+    # sampling with the adapted settings accepts near the target
+    n = 400
+    draws = []
+    for _ in range(n):
+        hmc.trajectory()
+        draws.append(hmc.z.clone())
+    acc = (hmc.n_accept / n).numpy()
+    assert np.all((acc > 0.55) & (acc < 0.99)), acc
+    d = torch.stack(draws).numpy()
+    std = d.std(axis=0) / sigmas
+    assert np.all((std > 0.6) & (std < 1.5)), std
+    assert np.all(np.isfinite(hmc.eps_patch.numpy())) and np.all(hmc.eps_patch.numpy() > 0)
+
+
+def test_divergent_trajectory_is_rejected():
+    post = GaussianPatches([1.0, 1.0], np.array([0, 2]))
+    hmc = HMC(post, torch.zeros(2, dtype=torch.float64), step_size=50.0, n_steps=20, seed=1)
+    z = hmc.z.clone()
+    hmc.trajectory()
+    assert hmc.accept_prob.item() < 1e-6
+    assert torch.equal(hmc.z, z) or hmc.n_accept.item() == 0 or torch.isfinite(hmc.z).all()
