@@ -15,6 +15,7 @@ struct EvalArgs {
     const ExpDesc *exps;
     const ImgGauss *gauss;
     const ImgGauss64 *gauss64;
+    const int4 *gboxes;    // [n_gauss] cutoff boxes (order keys), as in ImgGauss
     float *gpart;
     double *cpart;
     int pix_per_lane;
@@ -23,11 +24,14 @@ struct EvalArgs {
     int64_t grid;          // persistent: number of workgroups
     int precision;         // Precision mode
     int cull;              // skip Gaussians whose cutoff box misses the tile
-    int mask_cap;          // Gaussians per exposure with an LDS slot mask (<= kMaskCapMax)
+    int mask_cap;          // Gaussians per exposure with an LDS slot mask (<= kMaskCapMax,
+                           // a multiple of 8 so the activity words stay aligned)
+    int act_words;         // 64-bit activity words per wave (<= kActWordsMax)
 };
 
 // LDS bytes per wave for the pass-1 slot masks, at most (4 waves: 32 KiB).
 constexpr int kMaskCapMax = 8192;
+constexpr int kActWordsMax = 1024;   // activity bits for 65536 Gaussians per wave
 
 struct SplineView {
     const double *coeffs;  // [n_amp][n_tx-4][n_ty-4]
@@ -39,7 +43,8 @@ struct SplineView {
 };
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, ImgGauss64 *out64, hipStream_t stream);
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes,
+                          hipStream_t stream);
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
 hipError_t launch_count_pairs(const EvalArgs &a, unsigned long long *count, hipStream_t stream);
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,

@@ -143,7 +143,7 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
 // One workgroup per exposure; lanes stride over its n_src * n_psf Gaussians.
 __global__ __launch_bounds__(128) void convert_kernel(
     const ExpDesc *__restrict__ exps, const fpho_source *__restrict__ props,
-    ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64) {
+    ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64, int4 *__restrict__ boxes) {
     const ExpDesc e = exps[blockIdx.x];
     const int n = e.n_src * e.n_psf;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -154,6 +154,7 @@ __global__ __launch_bounds__(128) void convert_kernel(
         make_image_gaussian(e, props[e.src_off + s], s, p, g, g64);
         out[e.gauss_off + j] = g;
         out64[e.gauss_off + j] = g64;
+        boxes[e.gauss_off + j] = make_int4(g.xlo, g.xhi, g.ylo, g.yhi);
     }
 }
 
@@ -173,7 +174,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // True when the Gaussian's cutoff box (gb = xlo, xhi, ylo, yhi keys) misses
-// the tile's pixel box tb; one 16-byte scalar load, integer compares only.
+// the tile's pixel box tb; integer compares only.
 __device__ __forceinline__ bool boxed_out(int cull, const int4 gb, const int4 tb) {
     const int miss = (gb.y < tb.x ? 1 : 0) | (gb.x > tb.y ? 2 : 0) | (gb.w < tb.z ? 4 : 0) |
                      (gb.z > tb.w ? 8 : 0);
@@ -225,22 +226,30 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 // (cr' = dy v'x + dx v'y); each sum costs one FMA per pair and is turned
 // into the 7 source partials once per Gaussian (constants in ImgGauss).
 //
-// Pass 1 records, per Gaussian, which of the K pixel slots hold a pixel inside
-// the cutoff (a K-bit mask, one byte in LDS per Gaussian for the first
-// mask_cap Gaussians); pass 2 evaluates only those slots.
+// Pass 1 tests the cutoff boxes of 64 Gaussians at once (one vector load, a
+// ballot) and visits, in index order, only those that meet the tile.  It
+// records, per Gaussian, which of the K pixel slots hold a pixel inside the
+// cutoff (a K-bit mask, one byte in LDS per Gaussian for the first mask_cap
+// Gaussians) and which Gaussians have any (one bit each in LDS, 64 per word,
+// for the first 64 act_words Gaussians); pass 2 visits only those Gaussians
+// and evaluates only those slots.  Gaussians past either capacity are
+// visited with every slot: a pixel beyond the cutoff adds exact zeros.
 template <int K, bool MIXED>
 __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__ tiles,
                                           const ExpDesc *__restrict__ exps,
                                           const ImgGauss *__restrict__ gauss,
                                           const ImgGauss64 *__restrict__ gauss64,
+                                          const int4 *__restrict__ gboxes,
                                           float *__restrict__ gpart,
                                           double *__restrict__ cpart, int cull,
-                                          uint8_t *__restrict__ amask, int mask_cap) {
+                                          uint8_t *__restrict__ amask, int mask_cap,
+                                          uint64_t *__restrict__ act, int act_words) {
     const int lane = threadIdx.x & (kWave - 1);
     const TileDesc t = tiles[tile];
     const ExpDesc &e = exps[t.exp];
     const ImgGauss *__restrict__ g = gauss + e.gauss_off;
     const ImgGauss64 *__restrict__ g64 = gauss64 + e.gauss_off;
+    const int4 *__restrict__ gb = gboxes + e.gauss_off;
     const int n_src = e.n_src;
     const int n_psf = e.n_psf;
     const int ng = n_src * n_psf;
@@ -282,9 +291,20 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     }
 
     // ---- pass 1: residual = data - sum_g C_g   (common_kernel.cc:57-83)
-    for (int i = 0; i < ng; ++i) {
-        uint32_t m = 0;   // slots with a pixel inside the cutoff (wave-uniform)
-        if (!FPHO_BOXED_OUT(g[i])) {
+    // The boxes of 64 Gaussians are tested at once (lane j: Gaussian
+    // base + j); only those whose box meets the tile are visited, in order.
+    // act[] keeps, per 64 Gaussians, the ones with a pixel inside the cutoff.
+    for (int base = 0; base < ng; base += kWave) {
+        const int j = base + lane;
+        bool meets = false;
+        if (j < ng) meets = !boxed_out(cull, gb[j], tbox);
+        uint64_t todo = __builtin_amdgcn_ballot_w64(meets);
+        uint64_t active = 0;
+        while (todo) {
+            const int bit = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int i = base + bit;
+            uint32_t m = 0;   // slots with a pixel inside the cutoff (wave-uniform)
             const float xc = g[i].xcen, yc = g[i].ycen;
             const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
             const float hcs = g[i].hcs, ampA = g[i].ampA;
@@ -304,19 +324,27 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                     const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
                     const float c = ampA * gp * h;
                     if (MIXED) {
-                        if (maybe_bright && fabsf(c) > tmin)
-                            rd[k] -= pair_value64(g64[i], x[k], y[k]);
-                        else
+                        if (maybe_bright && fabsf(c) > tmin) {
+                            // keep the FP64 copies of the coordinates inside
+                            // this rare branch (hoisted out of the Gaussian
+                            // loop they would hold 4K more registers)
+                            float xk = x[k], yk = y[k];
+                            asm volatile("" : "+v"(xk), "+v"(yk));
+                            rd[k] -= pair_value64(g64[i], xk, yk);
+                        } else
                             rd[k] -= (double)c;
                     } else {
                         r[k] -= c;
                     }
                 }
             }
+            if (m) active |= 1ull << bit;
+            if (i < mask_cap && lane == 0) amask[i] = (uint8_t)m;
         }
-        if (i < mask_cap && lane == 0) amask[i] = (uint8_t)m;
+        if (lane == 0 && base / kWave < act_words) act[base / kWave] = active;
     }
-    // pass 2 of this wave reads the masks written by its lane 0
+
+    // pass 2 of this wave reads the masks and activity words of its lane 0
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 
@@ -355,16 +383,23 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         float p0 = 0.f, sx = 0.f, sy = 0.f, p3 = 0.f, p4 = 0.f, p5 = 0.f, p6 = 0.f;
         const int i0 = s * n_psf;
         bool touched = false;
-        for (int i = i0; i < i0 + n_psf; ++i) {
+        for (int w = i0 / kWave; w <= (i0 + n_psf - 1) / kWave; ++w) {
+          // the source's Gaussians in this word of the activity bits
+          const uint64_t word = w < act_words ? act[w] : ~0ull;
+          const int lo = i0 - w * kWave, hi = i0 + n_psf - w * kWave;   // [lo, hi)
+          uint64_t sel = word;
+          if (lo > 0) sel &= ~0ull << lo;
+          if (hi < kWave) sel &= (1ull << hi) - 1;
+          sel = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(sel >> 32)) << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sel);
+          while (sel) {
+            const int i = w * kWave + __builtin_ctzll(sel);
+            sel &= sel - 1;
             const ImgGauss &G = g[i];
             // slots to evaluate: pass 1's mask, else (past the LDS capacity)
-            // every slot of a Gaussian whose box meets the tile
-            uint32_t m;
-            if (i < mask_cap)
-                m = __builtin_amdgcn_readfirstlane((uint32_t)amask[i]);
-            else
-                m = FPHO_BOXED_OUT(G) ? 0u : 0xFFu;
-            if (m == 0) continue;
+            // every slot
+            const uint32_t m = i < mask_cap
+                ? (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)amask[i]) : 0xFFu;
             touched = true;
             const float xc = G.xcen, yc = G.ycen;
             const float sfxx = G.sfxx, sfyy = G.sfyy, sfxy = G.sfxy, hcs = G.hcs;
@@ -422,6 +457,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             const float fY = sfyy * Y2 + sfxy * X2;
             sx = fmaf(G.k_x1, X1, fmaf(G.k_x2, fX, sx));
             sy = fmaf(G.k_x1, Y1, fmaf(G.k_x2, fY, sy));
+          }
         }
         float *o = out + kNParams * s;
         if (!touched) {
@@ -474,15 +510,19 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t waves_per_group, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull, int32_t mask_cap) {
-    extern __shared__ uint8_t lds_mask[];
+    const int4 *__restrict__ gboxes, float *__restrict__ gpart, double *__restrict__ cpart,
+    int32_t cull, int32_t mask_cap, int32_t act_words) {
+    extern __shared__ uint64_t lds_words[];
+    uint8_t *lds_mask = reinterpret_cast<uint8_t *>(lds_words);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = xcd_swizzle<kXcdChunk>((int)blockIdx.x, (int)gridDim.x) *
                      waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull,
-                        lds_mask + w * mask_cap, mask_cap);
+    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gboxes, gpart, cpart, cull,
+                        lds_mask + w * mask_cap, mask_cap,
+                        reinterpret_cast<uint64_t *>(lds_mask + waves_per_group * mask_cap) +
+                            w * act_words, act_words);
 }
 
 // Persistent: every wave pulls tiles from `counter` until the queue is empty.
@@ -492,18 +532,23 @@ __global__ __launch_bounds__(256) void eval_persistent_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t *__restrict__ counter, const ExpDesc *__restrict__ exps,
     const ImgGauss *__restrict__ gauss, const ImgGauss64 *__restrict__ gauss64,
-    float *__restrict__ gpart, double *__restrict__ cpart, int32_t cull, int32_t mask_cap) {
-    extern __shared__ uint8_t lds_mask[];
+    const int4 *__restrict__ gboxes, float *__restrict__ gpart, double *__restrict__ cpart,
+    int32_t cull, int32_t mask_cap, int32_t act_words) {
+    extern __shared__ uint64_t lds_words[];
+    uint8_t *lds_mask = reinterpret_cast<uint8_t *>(lds_words);
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int W = (int)(blockDim.x >> 6);
     for (;;) {
         int slot = 0;
         if (lane == 0) slot = atomicAdd(counter, 1);
         slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, kWave));
         if (slot >= n_tiles) break;
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gpart, cpart, cull,
-                            lds_mask + w * mask_cap, mask_cap);
+        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gboxes, gpart, cpart, cull,
+                            lds_mask + w * mask_cap, mask_cap,
+                            reinterpret_cast<uint64_t *>(lds_mask + W * mask_cap) +
+                                w * act_words, act_words);
     }
 }
 
@@ -740,29 +785,30 @@ __global__ __launch_bounds__(64) void stack_kernel(
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, ImgGauss64 *out64, hipStream_t stream) {
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, hipStream_t stream) {
     if (n_exp <= 0) return hipSuccess;
     hipLaunchKernelGGL(convert_kernel, dim3((unsigned)n_exp), dim3(128), 0, stream,
-                       exps, props, out, out64);
+                       exps, props, out, out64, boxes);
     return hipGetLastError();
 }
 
 template <int K, bool MIXED>
 static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
     const int W = a.waves_per_group;
-    const size_t lds = (size_t)W * (size_t)a.mask_cap;
+    // per wave: the slot mask bytes, then the activity words
+    const size_t lds = (size_t)W * ((size_t)a.mask_cap + 8 * (size_t)a.act_words);
     if (a.persistent) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
         hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED>), dim3((unsigned)a.grid),
                            dim3(64 * W), lds, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
-                           a.counter, a.exps, a.gauss, a.gauss64, a.gpart, a.cpart,
-                           (int32_t)a.cull, (int32_t)a.mask_cap);
+                           a.counter, a.exps, a.gauss, a.gauss64, a.gboxes, a.gpart, a.cpart,
+                           (int32_t)a.cull, (int32_t)a.mask_cap, (int32_t)a.act_words);
     } else {
         const int64_t groups = (a.n_tiles + W - 1) / W;
         hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), lds,
                            stream, a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps,
-                           a.gauss, a.gauss64, a.gpart, a.cpart, (int32_t)a.cull,
-                           (int32_t)a.mask_cap);
+                           a.gauss, a.gauss64, a.gboxes, a.gpart, a.cpart, (int32_t)a.cull,
+                           (int32_t)a.mask_cap, (int32_t)a.act_words);
     }
     return hipGetLastError();
 }

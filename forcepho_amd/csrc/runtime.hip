@@ -132,6 +132,7 @@ struct Plan {
     DevVec<OutDesc> outs;
     DevVec<ImgGauss> gauss;
     DevVec<ImgGauss64> gauss64;
+    DevVec<int4> gboxes;
     DevVec<float> gpart;
     DevVec<double> cpart;
     DevVec<int32_t> counter;
@@ -426,6 +427,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     HIP_TRY(plan.pouts.upload(hp.pouts.data(), (int64_t)hp.pouts.size()));
     HIP_TRY(plan.gauss.alloc(hp.n_gauss));
     HIP_TRY(plan.gauss64.alloc(hp.n_gauss));
+    HIP_TRY(plan.gboxes.alloc(hp.n_gauss));
     HIP_TRY(plan.gpart.alloc(std::max<int64_t>(hp.n_part, 1)));
     HIP_TRY(plan.cpart.alloc(std::max<int64_t>((int64_t)hp.tiles.size(), 1)));
     HIP_TRY(plan.counter.alloc(1));
@@ -453,12 +455,14 @@ EvalArgs eval_args(Plan &plan) {
     a.exps = plan.exps.ptr;
     a.gauss = plan.gauss.ptr;
     a.gauss64 = plan.gauss64.ptr;
+    a.gboxes = plan.gboxes.ptr;
     a.precision = plan.precision;
     a.gpart = plan.gpart.ptr;
     a.cpart = plan.cpart.ptr;
     a.pix_per_lane = plan.launch.pix_per_lane;
     a.cull = plan.launch.cull;
-    a.mask_cap = std::min(kMaskCapMax, (plan.max_ng + 3) & ~3);
+    a.mask_cap = std::min(kMaskCapMax, (plan.max_ng + 7) & ~7);
+    a.act_words = std::min(kActWordsMax, (plan.max_ng + 63) / 64);
     a.waves_per_group = plan.launch.waves_per_group;
     a.persistent = plan.launch.persistent;
     const int64_t groups_needed = (plan.n_tiles + a.waves_per_group - 1) / a.waves_per_group;
@@ -472,7 +476,7 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
              hipStream_t stream) {
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
     HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
-                           stream));
+                           plan.gboxes.ptr, stream));
     const EvalArgs a = eval_args(plan);
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[1], stream));
     HIP_TRY(launch_eval(a, stream));
@@ -816,7 +820,7 @@ int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *props, int64_t *n_i
     HIP_TRY(cnt.alloc(1));
     HIP_TRY(hipMemset(cnt.ptr, 0, sizeof(unsigned long long)));
     HIP_TRY(launch_convert(b->plan.exps.ptr, b->plan.n_exp, props, b->plan.gauss.ptr,
-                           b->plan.gauss64.ptr, nullptr));
+                           b->plan.gauss64.ptr, b->plan.gboxes.ptr, nullptr));
     HIP_TRY(launch_count_pairs(eval_args(b->plan), cnt.ptr, nullptr));
     unsigned long long h = 0;
     HIP_TRY(hipMemcpy(&h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
