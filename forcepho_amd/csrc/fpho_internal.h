@@ -100,7 +100,7 @@ struct alignas(8) TileDesc {
     int32_t exp;       // ExpDesc index
     int32_t npix;      // <= 64*K
     int32_t pix0;      // offset into the patch pixel arrays
-    int32_t pad;
+    int32_t chunk0;    // first of the tile's 64-pixel slot boxes (plan-wide index)
     int64_t part_off;  // first float of this tile's gradient partials
     int32_t xlo, xhi, ylo, yhi;
 };

@@ -16,6 +16,7 @@ struct EvalArgs {
     const ImgGauss *gauss;
     const ImgGauss64 *gauss64;
     const int4 *gboxes;    // [n_gauss] cutoff boxes (order keys), as in ImgGauss
+    const int4 *slot_boxes;  // per tile (from TileDesc::chunk0), its 64-pixel slot boxes
     float *gpart;
     double *cpart;
     int pix_per_lane;

@@ -133,6 +133,7 @@ struct Plan {
     DevVec<ImgGauss> gauss;
     DevVec<ImgGauss64> gauss64;
     DevVec<int4> gboxes;
+    DevVec<int4> slot_boxes;
     DevVec<float> gpart;
     DevVec<double> cpart;
     DevVec<int32_t> counter;
@@ -293,6 +294,7 @@ int upload_meta(fpho_patch *p, const fpho_patch_desc *d) {
 struct HostPlan {
     std::vector<ExpDesc> exps;
     std::vector<TileDesc> tiles;
+    std::vector<int4> slot_boxes;   // per tile, the boxes of its 64-pixel slots
     std::vector<int64_t> cost;
     std::vector<OutDesc> outs;
     std::vector<SrcDesc> srcs;
@@ -349,7 +351,7 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
                 t.exp = ei;
                 t.npix = std::min(tile_pix, n_e - off);
                 t.pix0 = p->exposure_start[e] + off;
-                t.pad = 0;
+                t.chunk0 = (int32_t)hp.slot_boxes.size();
                 t.part_off = hp.n_part;
                 // union of the boxes of the tile's 64-pixel chunks
                 const int64_t c0 = p->chunk_start[e] + off / kWave;
@@ -358,6 +360,11 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
                 t.xhi = p->chunk_box[4 * c0 + 1];
                 t.ylo = p->chunk_box[4 * c0 + 2];
                 t.yhi = p->chunk_box[4 * c0 + 3];
+                for (int64_t c = c0; c < c1; ++c)
+                    hp.slot_boxes.push_back(make_int4(p->chunk_box[4 * c + 0],
+                                                      p->chunk_box[4 * c + 1],
+                                                      p->chunk_box[4 * c + 2],
+                                                      p->chunk_box[4 * c + 3]));
                 for (int64_t c = c0 + 1; c < c1; ++c) {
                     t.xlo = std::min(t.xlo, p->chunk_box[4 * c + 0]);
                     t.xhi = std::max(t.xhi, p->chunk_box[4 * c + 1]);
@@ -410,7 +417,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
         plan.theta_off.push_back(plan.theta_off.back() +
                                  (int64_t)p->n_sources * (p->n_bands + 6));
     }
-    if ((int64_t)hp.tiles.size() > INT32_MAX)
+    if ((int64_t)hp.tiles.size() > INT32_MAX || (int64_t)hp.slot_boxes.size() > INT32_MAX)
         return fail(FPHO_E_LIMIT, "too many tiles (%zu)", hp.tiles.size());
     // Queue order: most expensive tiles first (longest-processing-time first);
     // ties keep canonical order, so tiles of one exposure stay adjacent.
@@ -421,6 +428,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
 
     HIP_TRY(plan.exps.upload(hp.exps.data(), (int64_t)hp.exps.size()));
     HIP_TRY(plan.tiles.upload(hp.tiles.data(), (int64_t)hp.tiles.size()));
+    HIP_TRY(plan.slot_boxes.upload(hp.slot_boxes.data(), (int64_t)hp.slot_boxes.size()));
     HIP_TRY(plan.queue.upload(queue.data(), (int64_t)queue.size()));
     HIP_TRY(plan.outs.upload(hp.outs.data(), (int64_t)hp.outs.size()));
     HIP_TRY(plan.srcs.upload(hp.srcs.data(), (int64_t)hp.srcs.size()));
@@ -456,6 +464,7 @@ EvalArgs eval_args(Plan &plan) {
     a.gauss = plan.gauss.ptr;
     a.gauss64 = plan.gauss64.ptr;
     a.gboxes = plan.gboxes.ptr;
+    a.slot_boxes = plan.slot_boxes.ptr;
     a.precision = plan.precision;
     a.gpart = plan.gpart.ptr;
     a.cpart = plan.cpart.ptr;
