@@ -223,11 +223,13 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 // With v = v'/s, F v = (s F v')/s^2, dx vx = u'/s (u' = dx v'x), they are
 // linear in 12 pixel sums of a', a' v'x, e v'x, a' v'y, e v'y,
 // a' dx^2, a' dy^2, a' dx dy, e, e u', e w' (w' = dy v'y), e cr'
-// (cr' = dy v'x + dx v'y); each sum costs one FMA per pair and is turned
-// into the 7 source partials once per Gaussian (constants in ImgGauss).
+// (cr' = dy v'x + dx v'y); each sum costs one FMA per pair (e w' is kept as
+// e E' = e (u' + w') and e u' subtracted once) and is turned into the 7
+// source partials once per Gaussian (constants in ImgGauss).
 //
 // Pass 1 tests the cutoff boxes of 64 Gaussians at once (one vector load, a
-// ballot) and visits, in index order, only those that meet the tile.  It
+// ballot) against the boxes of the tile's 64-pixel slots and visits, in
+// index order, only those that meet a slot, at those slots.  It
 // records, per Gaussian, which of the K pixel slots hold a pixel inside the
 // cutoff (a K-bit mask, one byte in LDS per Gaussian for the first mask_cap
 // Gaussians) and which Gaussians have any (one bit each in LDS, 64 per word,
@@ -424,7 +426,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             const float sfxx = G.sfxx, sfyy = G.sfyy, sfxy = G.sfxy, hcs = G.hcs;
             // the 12 per-Gaussian pixel sums (over this lane's K pixels)
             float PA = 0.f, X1 = 0.f, X2 = 0.f, Y1 = 0.f, Y2 = 0.f, Pxx = 0.f;
-            float Pyy = 0.f, Pxy = 0.f, Pe = 0.f, Peu = 0.f, Pew = 0.f, Pecr = 0.f;
+            float Pyy = 0.f, Pxy = 0.f, Pe = 0.f, Peu = 0.f, Pea = 0.f, Pecr = 0.f;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if (!(m & (1u << k))) continue;
@@ -453,12 +455,12 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                 Pyy = fmaf(ay, dy, Pyy);
                 Pe += ee;
                 Peu = fmaf(ee, u, Peu);
-                Pew = fmaf(ee, ea - u, Pew);
+                Pea = fmaf(ee, ea, Pea);
                 Pecr = fmaf(ee, fmaf(dy, vx, dx * vy), Pecr);
             }
             // per-Gaussian epilogue: image-space derivative sums -> 7 partials
             const float gfx = G.k_axx * Pxx + G.k_e * Pe + G.k_u * Peu;
-            const float gfy = G.k_axx * Pyy + G.k_e * Pe + G.k_u * Pew;
+            const float gfy = G.k_axx * Pyy + G.k_e * Pe + G.k_u * (Pea - Peu);
             const float gfxy = G.k_axy * Pxy + G.k_u * Pecr;
             p0 = fmaf(G.cA_flux, PA, p0);
             p5 = fmaf(G.cA_n, PA, p5);
