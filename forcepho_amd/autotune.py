@@ -4,7 +4,11 @@ Candidates are (pixels per lane K, waves per workgroup W, persistent or
 grid launch, workgroups per CU for the persistent grid).  Each candidate is
 timed with HIP events on the evaluation stream over a few repetitions of a
 representative batch; the fastest is stored in a JSON cache keyed by device
-architecture and workload kind, which ``load_or_default`` reads at start-up.  Results
+architecture and workload kind, which ``load_or_default`` reads at start-up.
+Launches within ``TIE`` (relative) of the fastest count as equal, and the
+simplest of them wins: a grid launch before a persistent one (the persistent
+kernel holds more registers and spills more), then fewer waves per
+workgroup, then fewer workgroups per CU.  Results
 do not depend on the persistent/grid choice or on W (same tiles, same fixed
 reduction order); K changes the tiling and therefore only the rounding of
 the FP32 partial sums.
@@ -15,12 +19,13 @@ import json
 import os
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
-DEFAULT = {"batch": dict(pix_per_lane=4, waves_per_group=4, persistent=1, groups_per_cu=8,
+TIE = 0.01
+DEFAULT = {"batch": dict(pix_per_lane=8, waves_per_group=1, persistent=0, groups_per_cu=8,
                          cull=1),
            "patch": dict(pix_per_lane=4, waves_per_group=4, persistent=0, groups_per_cu=8,
                          cull=1)}
 
-__all__ = ["candidates", "tune", "load_or_default", "save"]
+__all__ = ["candidates", "pick", "tune", "load_or_default", "save"]
 
 
 def candidates(kind="batch"):
@@ -45,6 +50,16 @@ def _device_name():
         return "unknown"
 
 
+def pick(table, tie=TIE):
+    """The launch to keep from ``table`` = [(ms, launch), ...]: the simplest
+    launch within ``tie`` of the fastest (grid before persistent, then fewer
+    waves per workgroup, then fewer workgroups per CU)."""
+    best = min(ms for ms, _ in table)
+    close = [c for ms, c in table if ms <= best * (1.0 + tie)]
+    return min(close, key=lambda c: (c["persistent"], c["waves_per_group"],
+                                      c["groups_per_cu"]))
+
+
 def tune(batch, run, cands=None, reps=3, log=None):
     """Time ``run()`` (one evaluation on torch's current stream) for every
     candidate launch of ``batch``; return (best, table)."""
@@ -66,8 +81,9 @@ def tune(batch, run, cands=None, reps=3, log=None):
         if log:
             log(f"  {c}: {ms:.3f} ms")
     table.sort(key=lambda t: t[0])
-    batch.set_launch(**table[0][1])
-    return table[0][1], table
+    best = pick(table)
+    batch.set_launch(**best)
+    return best, table
 
 
 def save(kind, launch, ms=None):
