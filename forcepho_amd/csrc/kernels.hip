@@ -370,11 +370,15 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
     __builtin_amdgcn_wave_barrier();
 
     // ---- chi2 = sum (r^2 - d^2) ierr^2 in FP64 (compute_gaussians_kernel.cu:276-287)
+    // (pixel indices recomputed from an opaque copy of the lane id: kept
+    // from the load above, they would hold K registers across pass 1)
+    int lane2 = lane;
+    asm volatile("" : "+v"(lane2));
     double c2 = 0.0;
     float rr[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int p = k * kWave + lane;
+        const int p = k * kWave + lane2;
         float dk = 0.0f, iek = 0.0f;
         if (p < t.npix) {
             dk = e.data[t.pix0 + p];
@@ -394,6 +398,8 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
         }
         if (e.residual != nullptr && p < t.npix) e.residual[t.pix0 + p] = r[k];
         rr[k] = -2.0f * ((r[k] * iek) * iek);  // -2 r ierr^2
+        // finish this slot here: deferred, its loaded values would spill
+        asm volatile("" : "+v"(c2), "+v"(rr[k]));
     }
     c2 = wave_sum(c2);
     if (lane == 0) cpart[tile] = c2;
@@ -525,9 +531,9 @@ __device__ __forceinline__ int xcd_swizzle(int b, int n) {
 }
 
 // Eight waves per SIMD (the most a SIMD holds) caps the kernels at 64
-// VGPRs.  In eval_kernel at K = 8 the few values spilled for that are
-// stored and reloaded once per tile, between the two passes, outside every
-// Gaussian loop (measured: 5.7% faster than 6 waves at 76 VGPRs).
+// VGPRs (measured at K = 8: 5.7% faster than 6 waves at 76 VGPRs).
+// eval_kernel fits without spills; the persistent kernel at K = 8 spills a
+// few values once per tile, outside every Gaussian loop.
 #define FPHO_EVAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
 
 // Non-persistent: wave w of workgroup b evaluates queue[b*W + w].
