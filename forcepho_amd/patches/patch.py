@@ -21,50 +21,10 @@ import numpy as np
 
 from ..limits import PSF_COLS, psf_struct_dtype
 from ..sources import Galaxy, Scene
+from .wcs import AffineWCS, scale_at_sky
 
 __all__ = ["PatchBase", "Patch", "AffineWCS", "scale_at_sky",
            "superpixel_order", "pack_exposures"]
-
-
-class AffineWCS:
-    """A linear world coordinate system without distortion.
-
-    pixel = crpix + CD^-1 [(ra - ra0) cos(dec0), dec - dec0]  (0-based pixels)
-    where ``cd`` (degrees per pixel) plays the role of astropy's
-    ``pixel_scale_matrix``.
-    """
-
-    has_distortion = False
-
-    def __init__(self, crpix, crval, cd):
-        self.crpix = np.asarray(crpix, dtype=np.float64)
-        self.crval = np.asarray(crval, dtype=np.float64)
-        self.cd = np.asarray(cd, dtype=np.float64)
-        self._cdinv = np.linalg.inv(self.cd)
-
-    @property
-    def pixel_scale_matrix(self):
-        return self.cd
-
-    def world_to_pixel_values(self, ra, dec):
-        dra = (np.asarray(ra) - self.crval[0]) * np.cos(np.deg2rad(self.crval[1]))
-        ddec = np.asarray(dec) - self.crval[1]
-        ci = self._cdinv
-        x = ci[0, 0] * dra + ci[0, 1] * ddec + self.crpix[0]
-        y = ci[1, 0] * dra + ci[1, 1] * ddec + self.crpix[1]
-        return x, y
-
-
-def scale_at_sky(sky, wcs):
-    """(CW, D) at ``sky`` for a WCS without distortion
-    (forcepho/slow/stamp.py:245-254): D = (3600 CD)^-1 pixels per arcsec,
-    CW = 3600 D diag(cos dec, 1) pixels per degree."""
-    ra, dec = sky
-    W = np.eye(2)
-    W[0, 0] = np.cos(np.deg2rad(dec))
-    D_mat = np.linalg.inv(wcs.pixel_scale_matrix * 3600.0)
-    CW_mat = np.matmul(D_mat * 3600.0, W)
-    return CW_mat, D_mat
 
 
 def superpixel_order(nx, ny, sx, sy):
