@@ -77,3 +77,38 @@ def test_adapt_then_capture(post):
     acc = (hmc.n_accept / 62.0).cpu().numpy()
     assert np.all((acc > 0.3) & (acc <= 1.0)), acc
     assert torch.isfinite(hmc.z).all()
+
+
+def test_nuts_on_device_posterior(hip):
+    """NUTS transitions of several patches in lockstep on the device:
+    warm-up, then draws whose trees stop by U-turn (not at the depth cap)."""
+    from forcepho_amd.nuts import run_nuts
+    from helpers import make_small_patches
+    items = make_small_patches()
+    patches = [p for p, _, _ in items]
+    for p in patches:
+        p.send_to_device()
+    batch = PatchBatch(patches)
+    splines = DeviceSplines()
+    theta = np.concatenate([t for _, _, t in items])
+    lo, hi = [], []
+    for p, _, t in items:
+        a, b = synth.theta_bounds(t, len(p.band_N))
+        lo.append(a)
+        hi.append(b)
+    post = DevicePosterior(batch, splines, np.concatenate(lo), np.concatenate(hi))
+    z0 = post.inverse_transform(theta)
+    try:
+        draws, s = run_nuts(post, z0, n_draws=20, n_warmup=60, step_size=1e-4,
+                            max_treedepth=8, seed=4)
+        torch.cuda.synchronize()
+        assert torch.isfinite(draws).all() and torch.isfinite(s.lnp).all()
+        assert (s.tree_depth >= 1).all() and (s.tree_depth < 8).all()
+        assert (s.n_accept > 0).all()
+        assert not bool(s.divergent.any())
+        q = post.transform(s.z).cpu().numpy()
+        assert np.all(np.isfinite(q))
+    finally:
+        batch.free()
+        for p in patches:
+            p.free()
