@@ -1,4 +1,4 @@
-"""Minimal FITS headers and image HDUs (numpy only).
+"""Minimal FITS headers, image HDUs and binary tables (numpy only).
 
 The packers read exposure metadata as FITS headers: the metadata store keeps
 them as header strings (forcepho/patches/storage.py:302-522, written with
@@ -11,10 +11,14 @@ the FITS standard (version 4.0) those paths use:
   (quoted, ``''`` escapes a quote), logical (T/F), integer and real values;
   COMMENT / HISTORY / blank commentary cards; the END card;
 * files of 2880-byte blocks: a primary HDU and IMAGE extensions, selected
-  by index or EXTNAME; big-endian data of BITPIX 8, 16, 32, 64, -32, -64,
-  with BSCALE / BZERO applied on read.
+  by index (negative counts from the end) or EXTNAME; big-endian data of
+  BITPIX 8, 16, 32, 64, -32, -64, with BSCALE / BZERO applied on read;
+* BINTABLE extensions with fixed-width columns (TFORM codes L, B, I, J, K,
+  E, D, A, with repeat counts), read as numpy structured arrays (source
+  catalogs are stored this way).
 
-Long-string CONTINUE cards, tables and compressed HDUs are not supported.
+Long-string CONTINUE cards, variable-length and complex table columns,
+ASCII tables and compressed HDUs are not supported.
 """
 
 import numpy as np
@@ -25,6 +29,8 @@ BLOCK = 2880
 CARD = 80
 _COMMENTARY = ("COMMENT", "HISTORY", "")
 _BITPIX = {8: ">u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}
+_TFORM = {"L": "i1", "B": ">u1", "I": ">i2", "J": ">i4", "K": ">i8", "E": ">f4",
+          "D": ">f8"}
 
 
 def _parse_value(text):
@@ -261,14 +267,49 @@ def getheader(path, ext=0):
     return _select(hdus, ext)[0]
 
 
+def _table(hdr, blob, off):
+    """Structured array of a BINTABLE HDU's rows."""
+    fields = []
+    for k in range(1, int(hdr["TFIELDS"]) + 1):
+        form = str(hdr[f"TFORM{k}"]).strip().upper()
+        i = 0
+        while i < len(form) and form[i].isdigit():
+            i += 1
+        rep = int(form[:i]) if i else 1
+        code = form[i:i + 1]
+        name = str(hdr.get(f"TTYPE{k}", f"col{k}")).strip()
+        if code == "A":
+            fields.append((name, f"S{rep}"))
+        elif code in _TFORM:
+            dt = np.dtype(_TFORM[code])
+            fields.append((name, dt) if rep == 1 else (name, dt, (rep,)))
+        else:
+            raise NotImplementedError(f"table column format {form!r} is not supported")
+    dt = np.dtype(fields)
+    if dt.itemsize != int(hdr["NAXIS1"]):
+        raise ValueError("BINTABLE row size does not match its columns")
+    rows = np.frombuffer(blob, dtype=dt, count=int(hdr["NAXIS2"]), offset=off)
+    native = []
+    for n in dt.names:
+        f = dt.fields[n][0]
+        native.append((n, f.newbyteorder("=") if f.base.kind in "iuf" else f))
+    out = np.empty(rows.shape, dtype=np.dtype(native))
+    for n in dt.names:
+        out[n] = rows[n]
+    return out
+
+
 def getdata(path, ext=0):
-    """Data of image HDU ``ext`` as a native-endian array of shape
-    (NAXISn, ..., NAXIS1); BSCALE / BZERO are applied (float result)."""
+    """Data of HDU ``ext`` (index or EXTNAME): an image as a native-endian
+    array of shape (NAXISn, ..., NAXIS1) with BSCALE / BZERO applied (float
+    result), a binary table as a structured array of its rows."""
     hdus, blob = _read_hdus(path)
     hdr, off, nbytes = _select(hdus, ext)
     naxis = int(hdr.get("NAXIS", 0))
     if naxis == 0:
         return None
+    if str(hdr.get("XTENSION", "")).strip() == "BINTABLE":
+        return _table(hdr, blob, off)
     shape = tuple(int(hdr[f"NAXIS{k}"]) for k in range(naxis, 0, -1))
     dt = np.dtype(_BITPIX[int(hdr["BITPIX"])])
     a = np.frombuffer(blob, dtype=dt, count=int(np.prod(shape)), offset=off)
