@@ -17,12 +17,20 @@ warm-up draws): Nesterov dual averaging of each patch's step size (Hoffman &
 Gelman 2014, Algorithm 5) and a regularised diagonal inverse mass matrix from
 the draws of a middle window, after which the step size adaptation restarts.
 Patches are independent chains, so every patch adapts its own step size.
+
+The mass matrix is diagonal by default (``inv_mass`` per parameter) or,
+with ``dense=True``, a full matrix per patch (forcepho/fitting.py:370-412,
+``get_pot(full=True)``): the inverse mass blocks are kept padded to the
+largest patch, [n_patches, m, m], momenta are drawn through the Cholesky
+factor of each block's inverse, and the warm-up windows estimate each
+patch's covariance (regularised toward the identity as Stan's dense
+adaptation does).
 """
 
 import numpy as np
 import torch
 
-__all__ = ["DevicePosterior", "HMC", "segment_sum_index"]
+__all__ = ["DevicePosterior", "HMC", "DenseMass", "segment_sum_index"]
 
 
 def segment_sum_index(theta_off):
@@ -73,12 +81,56 @@ class DevicePosterior:
         return lnp, grad
 
 
+class DenseMass:
+    """Per-patch dense inverse mass matrices over the flat parameter vector
+    of a posterior (``seg_idx``/``seg_mask`` from :func:`segment_sum_index`)."""
+
+    def __init__(self, post, inv_mass_blocks=None):
+        self.idx = post.seg_idx
+        self.mask = post.seg_mask
+        P, m = self.idx.shape
+        dev = self.mask.device
+        eye = torch.eye(m, dtype=torch.float64, device=dev).expand(P, m, m)
+        self.minv = eye.clone() if inv_mass_blocks is None else \
+            torch.as_tensor(inv_mass_blocks, dtype=torch.float64, device=dev).clone()
+        self.chol = torch.empty_like(self.minv)
+        # flat parameter j sits at position pos[j] of the [P*m] padded blocks
+        idx = self.idx.cpu().numpy()
+        msk = self.mask.cpu().numpy() > 0
+        pos = np.zeros(int(idx[msk].max()) + 1, dtype=np.int64)
+        pp, kk = np.nonzero(msk)
+        pos[idx[pp, kk]] = pp * m + kk
+        self.pos = torch.as_tensor(pos, device=dev)
+        self._pad_eye = torch.diag_embed(1.0 - self.mask)
+        self.update(self.minv)
+
+    def update(self, minv):
+        """Set the inverse mass blocks; padded rows and columns become the
+        identity, and the momentum factor chol(M) is recomputed."""
+        mm = self.mask[:, :, None] * self.mask[:, None, :]
+        self.minv.copy_(minv * mm + self._pad_eye)
+        self.chol.copy_(torch.linalg.cholesky(torch.linalg.inv(self.minv)))
+
+    def blocks(self, v):
+        return v[self.idx] * self.mask
+
+    def flat(self, b):
+        return b.reshape(-1)[self.pos]
+
+    def draw(self, like):
+        e = torch.randn(self.idx.shape, dtype=torch.float64, device=like.device)
+        return self.flat(torch.matmul(self.chol, e[..., None])[..., 0])
+
+    def velocity(self, p):
+        return self.flat(torch.matmul(self.minv, self.blocks(p)[..., None])[..., 0])
+
+
 class HMC:
-    """Fixed-length HMC with a diagonal mass matrix, a step size per patch and
-    per-patch acceptance."""
+    """Fixed-length HMC with a diagonal (or, with ``dense``, a full
+    per-patch) mass matrix, a step size per patch and per-patch acceptance."""
 
     def __init__(self, posterior, z0, step_size=1e-3, n_steps=10, seed=0, inv_mass=None,
-                 jitter=0.2):
+                 jitter=0.2, dense=False):
         self.post = posterior
         # each trajectory draws its step sizes uniformly in eps (1 +- jitter),
         # so a fixed-length trajectory cannot lock onto a period of the target
@@ -92,14 +144,28 @@ class HMC:
         self.n_accept = torch.zeros_like(self.lnp)
         self.accept_prob = torch.zeros_like(self.lnp)   # of the last trajectory
         self.eps_patch = torch.full_like(self.lnp, float(step_size))
-        self.inv_mass = (torch.ones_like(self.z) if inv_mass is None else
+        self.inv_mass = (torch.ones_like(self.z) if inv_mass is None or dense else
                          torch.as_tensor(inv_mass, dtype=torch.float64, device=dev).clone())
+        self.dense = DenseMass(posterior, inv_mass) if dense else None
         # the default device generator is the one torch graphs replay safely
         if dev.type == "cuda":
             torch.cuda.manual_seed(int(seed))
         else:
             torch.manual_seed(int(seed))
         self.graph = None
+
+    # -- mass matrix ------------------------------------------------------
+    def _draw_p(self):
+        if self.dense is not None:
+            return self.dense.draw(self.z)
+        p = torch.randn(self.z.shape, dtype=torch.float64, device=self.z.device)
+        return p * torch.rsqrt(self.inv_mass)
+
+    def _velocity(self, p):
+        return self.dense.velocity(p) if self.dense is not None else self.inv_mass * p
+
+    def _kinetic(self, p):
+        return 0.5 * self.post.segment_sum(p * self._velocity(p))
 
     @property
     def eps(self):
@@ -113,7 +179,7 @@ class HMC:
         p = p + 0.5 * eps * grad
         lnp = None
         for i in range(self.n_steps):
-            z = z + eps * self.inv_mass * p
+            z = z + eps * self._velocity(p)
             lnp, grad = self.post(z)
             p = p + (eps if i < self.n_steps - 1 else 0.5 * eps) * grad
         return z, p, lnp, grad
@@ -122,15 +188,14 @@ class HMC:
         """One trajectory and per-patch Metropolis step; updates the state
         tensors in place (graph-capturable)."""
         post = self.post
-        p0 = torch.randn(self.z.shape, dtype=torch.float64, device=self.z.device)
-        p0 = p0 * torch.rsqrt(self.inv_mass)
-        h0 = -self.lnp + 0.5 * post.segment_sum(p0 * p0 * self.inv_mass)
+        p0 = self._draw_p()
+        h0 = -self.lnp + self._kinetic(p0)
         eps = self.eps_patch
         if self.jitter > 0.0:
             u = torch.rand(eps.shape, dtype=torch.float64, device=eps.device)
             eps = eps * (1.0 + self.jitter * (2.0 * u - 1.0))
         z1, p1, lnp1, grad1 = self.leapfrog(self.z, p0, self.grad, eps[post.param_patch])
-        h1 = -lnp1 + 0.5 * post.segment_sum(p1 * p1 * self.inv_mass)
+        h1 = -lnp1 + self._kinetic(p1)
         dh = h0 - h1
         # a diverging trajectory (non-finite energy) is rejected
         dh = torch.where(torch.isfinite(dh), dh, torch.full_like(dh, -torch.inf))
@@ -149,9 +214,9 @@ class HMC:
         toward ``target_accept`` throughout; between an initial and a final
         fast phase (fractions ``fast`` of the warm-up), windows of doubling
         length (from ``first_window``) each end by setting the inverse mass
-        matrix to the per-parameter variance of the window's draws
-        (shrunk toward 1e-3 of the patch's mean variance) and restarting the
-        step size adaptation.  Ends with each patch's averaged step size.
+        matrix to the per-parameter variance of the window's draws (with a
+        dense mass, each patch's covariance), shrunk toward 1e-3 of the
+        patch's mean variance, and restarting the step size adaptation.  Ends with each patch's averaged step size.
         Runs eagerly; capture afterwards."""
         n_warmup = int(n_warmup)
         start = int(fast[0] * n_warmup)
@@ -174,6 +239,10 @@ class HMC:
         mu, hbar, log_eps_bar, m = restart()
         mean = torch.zeros_like(self.z)
         m2 = torch.zeros_like(self.z)
+        if self.dense is not None:
+            P, m = self.dense.idx.shape
+            cmean = torch.zeros((P, m), dtype=torch.float64, device=self.z.device)
+            cm2 = torch.zeros((P, m, m), dtype=torch.float64, device=self.z.device)
         n_mass = 0
         for it in range(n_warmup):
             self.trajectory()
@@ -189,12 +258,24 @@ class HMC:
                 delta = self.z - mean
                 mean += delta / n_mass
                 m2 += delta * (self.z - mean)
+                if self.dense is not None:
+                    zb = self.dense.blocks(self.z)
+                    db = zb - cmean
+                    cmean += db / n_mass
+                    cm2 += db[:, :, None] * (zb - cmean)[:, None, :]
                 if it in ends and n_mass > 2:
                     var = m2 / (n_mass - 1)
                     scale = self.post.segment_sum(var) / self.post.segment_sum(
                         torch.ones_like(var))
-                    self.inv_mass.copy_((n_mass / (n_mass + 5.0)) * var + 1e-3 *
-                                        (5.0 / (n_mass + 5.0)) * scale[self.post.param_patch])
+                    w = n_mass / (n_mass + 5.0)
+                    reg = 1e-3 * (5.0 / (n_mass + 5.0))
+                    self.inv_mass.copy_(w * var + reg * scale[self.post.param_patch])
+                    if self.dense is not None:
+                        cov = cm2 / (n_mass - 1)
+                        eye = torch.eye(cov.shape[-1], dtype=cov.dtype, device=cov.device)
+                        self.dense.update(w * cov + reg * scale[:, None, None] * eye)
+                        cmean.zero_()
+                        cm2.zero_()
                     mu, hbar, log_eps_bar, m = restart()
                     mean.zero_()
                     m2.zero_()

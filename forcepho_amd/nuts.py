@@ -25,7 +25,8 @@ A chain whose tree has stopped keeps its state while the others continue;
 the batch stops doubling when every chain has stopped or at
 ``max_treedepth``.  The step-size and mass-matrix warm-up is
 :meth:`forcepho_amd.hmc.HMC.adapt`, driven by the mean acceptance statistic
-of each tree's leaves.  The mass matrix is diagonal.
+of each tree's leaves.  The mass matrix is diagonal or, with
+``dense=True``, a full matrix per patch (``full=True`` of ``run_lmc``).
 """
 
 import numpy as np
@@ -47,7 +48,8 @@ def _ckpt_range(n):
 
 class NUTS(HMC):
     """Multinomial NUTS for the chains of a :class:`DevicePosterior` (one
-    chain per patch), diagonal inverse mass ``inv_mass``.
+    chain per patch), with a diagonal (``inv_mass``) or, with ``dense``, a
+    full per-patch inverse mass matrix.
 
     ``trajectory()`` draws one sample per chain; ``tree_depth`` and
     ``n_leapfrog`` hold the last tree's depth and leaf count per chain,
@@ -58,9 +60,9 @@ class NUTS(HMC):
     """
 
     def __init__(self, posterior, z0, step_size=1e-3, max_treedepth=10, seed=0,
-                 inv_mass=None, jitter=0.0, max_delta_h=1000.0):
+                 inv_mass=None, jitter=0.0, max_delta_h=1000.0, dense=False):
         super().__init__(posterior, z0, step_size=step_size, n_steps=1, seed=seed,
-                         inv_mass=inv_mass, jitter=jitter)
+                         inv_mass=inv_mass, jitter=jitter, dense=dense)
         self.max_treedepth = int(max_treedepth)
         self.max_delta_h = float(max_delta_h)
         self.tree_depth = torch.zeros_like(self.lnp)
@@ -71,20 +73,17 @@ class NUTS(HMC):
         raise NotImplementedError("NUTS trees have data-dependent length; run eagerly")
 
     # -- helpers --------------------------------------------------------
-    def _kinetic(self, p):
-        return 0.5 * self.post.segment_sum(p * p * self.inv_mass)
-
     def _turning(self, p_start, p_end, rho):
         """Generalised U-turn criterion per chain with the end momenta
         counted half (trapezoidal rule): turning when the momentum sum,
         mapped by M^-1, points against either end's momentum."""
         r = rho - 0.5 * (p_start + p_end)
         seg = self.post.segment_sum
-        return (seg(self.inv_mass * p_start * r) <= 0) | (seg(self.inv_mass * p_end * r) <= 0)
+        return (seg(self._velocity(p_start) * r) <= 0) | (seg(self._velocity(p_end) * r) <= 0)
 
     def _leapfrog1(self, z, p, grad, eps):
         p = p + 0.5 * eps * grad
-        z = z + eps * self.inv_mass * p
+        z = z + eps * self._velocity(p)
         lnp, grad = self.post(z)
         p = p + 0.5 * eps * grad
         return z, p, lnp, grad
@@ -97,7 +96,7 @@ class NUTS(HMC):
         f64 = torch.float64
         nchain = self.lnp.shape[0]
 
-        p0 = torch.randn(self.z.shape, dtype=f64, device=dev) * torch.rsqrt(self.inv_mass)
+        p0 = self._draw_p()
         h0 = -self.lnp + self._kinetic(p0)
         eps_c = self.eps_patch
         if self.jitter > 0.0:
@@ -205,12 +204,12 @@ class NUTS(HMC):
 
 
 def run_nuts(posterior, z0, n_draws, n_warmup=500, step_size=1e-3, max_treedepth=10,
-             target_accept=0.8, seed=0, inv_mass=None):
+             target_accept=0.8, seed=0, inv_mass=None, dense=False):
     """Warm up and draw ``n_draws`` samples of every chain (the role of
     ``run_lmc``, forcepho/fitting.py:255-367).  Returns (draws
     [n_draws, n_params] in the unconstrained space, the sampler)."""
     s = NUTS(posterior, z0, step_size=step_size, max_treedepth=max_treedepth, seed=seed,
-             inv_mass=inv_mass)
+             inv_mass=inv_mass, dense=dense)
     if n_warmup > 0:
         s.adapt(n_warmup, target_accept=target_accept)
     out = torch.empty((int(n_draws),) + tuple(s.z.shape), dtype=torch.float64,

@@ -112,3 +112,21 @@ def test_nuts_on_device_posterior(hip):
         batch.free()
         for p in patches:
             p.free()
+
+
+def test_dense_mass_adapt_then_capture(post):
+    """Dense (per-patch full) mass matrix: warm-up, then graph-captured
+    trajectories, with every replay's state finite and the chain moving."""
+    p, theta = post
+    z0 = p.inverse_transform(theta)
+    hmc = HMC(p, z0, step_size=1e-5, n_steps=10, seed=3, dense=True)
+    hmc.adapt(150, target_accept=0.8)
+    minv = hmc.dense.minv
+    assert torch.isfinite(minv).all()
+    assert (torch.linalg.eigvalsh(minv) > 0).all()
+    hmc.capture(warmup=2)
+    hmc.run(40)
+    torch.cuda.synchronize()
+    acc = (hmc.n_accept / 42.0).cpu().numpy()
+    assert np.all((acc > 0.3) & (acc <= 1.0)), acc
+    assert torch.isfinite(hmc.z).all() and torch.isfinite(hmc.lnp).all()

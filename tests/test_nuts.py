@@ -114,3 +114,48 @@ def test_nuts_stops_on_divergence():
     # a diverged first subtree leaves the chain where it was
     if float(s.tree_depth[0]) == 0:
         assert torch.equal(s.z, z_before)
+
+
+def test_dense_mass_blocks_match_explicit_matrix():
+    from forcepho_amd.hmc import DenseMass
+    rng = np.random.default_rng(4)
+    sizes = (3, 1, 2)
+    blocks = []
+    for m in sizes:
+        a = rng.normal(size=(m, m))
+        blocks.append(a @ a.T + m * np.eye(m))
+    post = GaussianPatches([np.eye(m) for m in sizes])
+    mmax = max(sizes)
+    pad = np.stack([np.pad(b, ((0, mmax - b.shape[0]), (0, mmax - b.shape[0])))
+                    for b in blocks])
+    dm = DenseMass(post, pad)
+    full = torch.block_diag(*[torch.as_tensor(b) for b in blocks])
+    p = torch.as_tensor(rng.normal(size=6))
+    assert torch.allclose(dm.velocity(p), full @ p)
+    # momenta ~ N(0, M) with M the inverse of the blocks
+    torch.manual_seed(0)
+    draws = torch.stack([dm.draw(p) for _ in range(20000)])
+    cov = np.cov(draws.numpy().T)
+    m_full = np.linalg.inv(full.numpy())
+    assert np.allclose(cov, m_full, atol=0.05 * np.abs(m_full).max())
+
+
+def test_dense_adaptation_learns_correlation_and_shortens_trees():
+    c = np.array([[4.0, 1.9], [1.9, 1.0]])          # correlation 0.95
+    post = GaussianPatches([np.linalg.inv(c)])
+    z0 = torch.zeros(2, dtype=torch.float64)
+    depth = {}
+    for dense in (False, True):
+        draws, s = run_nuts(post, z0, n_draws=800, n_warmup=400, step_size=0.1, seed=7,
+                            dense=dense)
+        d = draws.numpy()
+        assert np.allclose(np.cov(d.T), c, rtol=0.2, atol=0.1), (dense, np.cov(d.T))
+        depth[dense] = []
+        for _ in range(200):
+            s.trajectory()
+            depth[dense].append(float(s.tree_depth[0]))
+        if dense:
+            minv = s.dense.minv[0].numpy()
+            assert np.allclose(minv, c, rtol=0.35, atol=0.2), minv
+    # with the correlation in the mass matrix the target is isotropic
+    assert np.mean(depth[True]) < np.mean(depth[False])
