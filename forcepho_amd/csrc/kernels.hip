@@ -6,7 +6,8 @@
 //   eval_kernel<K>      per-pixel mixture, residual, chi2 and the 7 gradient
 //                       partials per source (common_kernel.cc:57-143,
 //                       compute_gaussians_kernel.cu:262-301); one wave per
-//                       tile of 64*K pixels, no LDS, no float atomics.
+//                       tile of 64*K pixels, no float atomics; LDS holds only
+//                       each wave's own slot masks and activity bits.
 //   reduce_kernel       fixed-order sum of tile partials (FP64 or FP32)
 //                       (replaces the shared-memory Accumulator,
 //                        compute_gaussians_kernel.cu:133-198)
@@ -204,7 +205,8 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 }
 
 // Evaluate one wave tile.  `tile` must be wave-uniform (an SGPR value) so
-// every descriptor and Gaussian load below is a scalar load.
+// every descriptor and Gaussian record load below is a scalar load (the
+// box scan alone reads 64 boxes with one vector load).
 //   MIXED = true : residual accumulated in FP64; a term whose FP32 value
 //                  exceeds kBrightSigma * sigma of the tile's quietest pixel
 //                  is re-evaluated in FP64 (a wave branch taken only near
