@@ -13,8 +13,9 @@ Usage (each ``<dir>`` is a rocprofv3 ``-d`` output directory):
         --cal-fetch <dir> --cal-write <dir> --launch K,W,P,G --command "..."
 
 ``traffic`` prints the summary that ``bench.py`` reads for ``roofline.traffic``.
-It carries a digest of ``csrc/kernels.hip``, so a summary taken on other
-kernel source is recognised as stale.
+It carries a digest of the kernel sources' code (``csrc/kernels.hip`` and
+the two headers it compiles with), so a summary taken on other kernel code
+is recognised as stale.
 """
 
 import argparse
@@ -22,20 +23,37 @@ import collections
 import glob
 import hashlib
 import os
+import re
 import sqlite3
 import statistics
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KERNEL_SOURCE = os.path.join(HERE, "csrc", "kernels.hip")
+# the sources whose code fixes the pixel-pass machine code
+KERNEL_SOURCES = [KERNEL_SOURCE, os.path.join(HERE, "csrc", "kernels.h"),
+                  os.path.join(HERE, "csrc", "fpho_internal.h")]
 COPY_BYTES = 1 << 30   # bench.py --copy-calibration copies 1 GiB per dispatch
 
 __all__ = ["kernel_source_digest", "kernel_stats", "counter_medians", "read_summary"]
 
 
-def kernel_source_digest():
-    """First 16 hex digits of the SHA-256 of the kernel source."""
-    with open(KERNEL_SOURCE, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:16]
+def _code_only(text):
+    """C/C++ source without comments, whitespace runs collapsed."""
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return " ".join(text.split())
+
+
+def kernel_source_digest(paths=None):
+    """First 16 hex digits of the SHA-256 of the kernel sources' code
+    (``kernels.hip``, ``kernels.h``, ``fpho_internal.h``; comments and
+    whitespace do not count, so a profile stays valid across comment edits)."""
+    h = hashlib.sha256()
+    for p in paths or KERNEL_SOURCES:
+        with open(p) as f:
+            h.update(_code_only(f.read()).encode())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
 
 
 def _dbs(path):

@@ -41,3 +41,15 @@ def test_kernel_stats_and_summary_round_trip(prof_dir, tmp_path):
     kv = profiling.read_summary(str(f))
     assert kv == {"launch": "8,1,0,8", "traffic_bytes_per_launch": "1.5e+10"}
     assert len(profiling.kernel_source_digest()) == 16
+
+
+def test_kernel_digest_ignores_comments_and_whitespace(tmp_path):
+    a = tmp_path / "a.hip"
+    b = tmp_path / "b.hip"
+    c = tmp_path / "c.hip"
+    a.write_text("// header\nint f(int x) { return x + 1; }  /* note */\n")
+    b.write_text("/* other\n   note */ int f(int x) {\n    return x + 1;   // same code\n}\n")
+    c.write_text("int f(int x) { return x + 2; }\n")
+    d = profiling.kernel_source_digest
+    assert d([str(a)]) == d([str(b)])
+    assert d([str(a)]) != d([str(c)])
