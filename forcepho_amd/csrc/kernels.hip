@@ -342,8 +342,9 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                 const bool in = !(ea < kCutScaled);       // E <= 36 (cutoff)
                 if (__builtin_amdgcn_ballot_w64(in) == 0) continue;
                 m |= 1u << k;
-                if (in) {
-                    const float gp = __builtin_amdgcn_exp2f(ea);
+                {
+                    // a lane beyond the cutoff adds an exact zero
+                    const float gp = __builtin_amdgcn_exp2f(in ? ea : -__builtin_huge_valf());
                     const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
                     const float c = ampA * gp * h;
                     if (MIXED) {
