@@ -330,7 +330,9 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             const float xc = g[i].xcen, yc = g[i].ycen;
             const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
             const float hcs = g[i].hcs, ampA = g[i].ampA;
-            const bool maybe_bright = g[i].peak > tmin;
+            // a term is bright when |c| > tb: a Gaussian whose peak stays
+            // below the tile threshold cannot have one
+            const float tb = g[i].peak > tmin ? tmin : __builtin_huge_valf();
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if (!(smi & (1u << k))) continue;
@@ -348,7 +350,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                     const float h = fmaf(vx, vx, fmaf(vy, vy, hcs));
                     const float c = ampA * gp * h;
                     if (MIXED) {
-                        if (maybe_bright && fabsf(c) > tmin) {
+                        if (fabsf(c) > tb) {
                             // keep the FP64 copies of the coordinates inside
                             // this rare branch (hoisted out of the Gaussian
                             // loop they would hold 4K more registers)
