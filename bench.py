@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--issue-calibration", action="store_true",
+                    help="only measure the VALU issue rate of FP32 FMA (and FMA with "
+                         "v_exp_f32) chains at 8 waves per SIMD, and exit")
     return ap.parse_args()
 
 
@@ -72,6 +75,30 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 5}), flush=True)
+
+
+def issue_calibration():
+    """Wave64 VALU instructions per SIMD per cycle of independent FP32 FMA
+    chains (8-byte and 4-byte encodings), and of FMA chains interleaved with
+    v_exp_f32 (library kernels; the shader clock is the device's nominal
+    one)."""
+    import ctypes
+    from forcepho_amd import _lib
+    _lib.init(0)
+    cus = _lib.cu_count()
+    out = {}
+    for kind, name in ((0, "fma_f32"), (2, "fmac_f32"), (1, "fma_f32+exp_f32")):
+        ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _lib.check(_lib.lib().fpho_issue_calibration(kind, ctypes.byref(ms), ctypes.byref(wi),
+                                                     ctypes.byref(clk)))
+        cycles = ms.value * 1e-3 * clk.value * 1e3
+        out[name] = {"ms": ms.value, "wave_instructions": wi.value,
+                     "clock_mhz": clk.value / 1e3,
+                     "per_simd_per_cycle": wi.value / (4 * cus * cycles)}
+        if kind in (0, 2):
+            # FP32 FMA = 2 FLOP per lane
+            out[name]["tflops"] = wi.value * 64 * 2 / (ms.value * 1e-3) / 1e12
+    print(json.dumps({"issue_calibration": out, "cus": cus}), flush=True)
 
 
 def measured_traffic(launch):
@@ -207,6 +234,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.issue_calibration:
+        issue_calibration()
         return
     if args.config == 5 and args.gpus == 1:
         run_config5(args)

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBNAME = "libfpho_hip.so"
 LIBPATH = os.path.join(HERE, LIBNAME)
-SOURCES = ["kernels.hip", "runtime.hip"]
+SOURCES = ["kernels.hip", "runtime.hip", "calibrate.hip"]
 ARCH = "gfx950"
 
 

@@ -30,7 +30,7 @@ EXPORTED = [
     "fpho_batch_create", "fpho_batch_layout", "fpho_batch_info",
     "fpho_batch_set_launch", "fpho_batch_set_precision",
     "fpho_batch_evaluate_async", "fpho_batch_set_timing", "fpho_batch_last_timing",
-    "fpho_batch_count_pairs",
+    "fpho_batch_count_pairs", "fpho_issue_calibration",
     "fpho_batch_free",
     "fpho_spline_create", "fpho_spline_free",
     "fpho_batch_theta_to_proposals_async", "fpho_batch_stack_grad_async",
@@ -99,6 +99,8 @@ def _declare(lib):
         "fpho_batch_set_timing": ([P, c_int], c_int),
         "fpho_batch_last_timing": ([P, c_float_p], c_int),
         "fpho_batch_count_pairs": ([P, P, c_i64_p], c_int),
+        "fpho_issue_calibration": ([c_int, c_double_p, c_double_p,
+                                    ctypes.POINTER(c_int)], c_int),
         "fpho_batch_free": ([P], c_int),
         "fpho_spline_create": ([c_double_p, c_int, c_double_p, c_int, c_double_p, c_int,
                                 c_double_p, ctypes.c_double, ctypes.POINTER(P)], c_int),

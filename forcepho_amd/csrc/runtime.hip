@@ -20,6 +20,7 @@
 
 #include "../../include/fpho.h"
 #include "fpho_internal.h"
+#include "calibrate.h"
 #include "kernels.h"
 
 using namespace fpho;
@@ -818,6 +819,42 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
     HIP_TRY(hipEventSynchronize(b->plan.ev[3]));
     for (int i = 0; i < 3; ++i)
         HIP_TRY(hipEventElapsedTime(&ms3[i], b->plan.ev[i], b->plan.ev[i + 1]));
+    return FPHO_OK;
+}
+
+int fpho_issue_calibration(int kind, double *ms, double *wave_instructions, int *clock_khz) {
+    if (!ms || !wave_instructions || !clock_khz) return fail(FPHO_E_ARG, "null argument");
+    if (kind < 0 || kind > 2) return fail(FPHO_E_ARG, "kind must be 0, 1 or 2 (got %d)", kind);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    int cus = 0, clk = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev));
+    const int blocks = 8 * cus;   // 8 workgroups of 4 waves: 8 waves per SIMD
+    DevVec<float> out;
+    HIP_TRY(out.alloc((int64_t)blocks * 256));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int64_t per_wave = 0;
+    float t = 0.0f;
+    hipError_t err = hipSuccess;
+    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {
+        if ((err = hipEventRecord(e0, nullptr)) != hipSuccess) break;
+        if ((err = launch_issue_calibration(kind, blocks, out.ptr, nullptr, &per_wave)) !=
+            hipSuccess)
+            break;
+        if ((err = hipEventRecord(e1, nullptr)) != hipSuccess) break;
+        if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+        err = hipEventElapsedTime(&t, e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess)
+        return fail(FPHO_E_HIP, "issue calibration failed: %s", hipGetErrorString(err));
+    *ms = t;
+    *wave_instructions = (double)blocks * 4.0 * (double)per_wave;
+    *clock_khz = clk;
     return FPHO_OK;
 }
 

@@ -198,6 +198,15 @@ int fpho_batch_set_timing(fpho_batch *b, int on);
 int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *proposals_dev,
                            int64_t *n_in);
 int fpho_batch_last_timing(fpho_batch *b, float *ms3);
+/* Measurement only, no counterpart in forcepho: VALU issue calibration of
+ * the current device.  kind 0 times independent FP32 FMA chains, kind 1 FMA
+ * chains interleaved with v_exp_f32, kind 2 FMA chains in the accumulate
+ * (v_fmac) encoding, at 8 waves per SIMD on every CU.
+ * Returns the kernel time (ms, last of three runs), the wave64 VALU
+ * instructions it issued and the device's shader clock (kHz), from which
+ * instructions per SIMD per cycle follow.  Synchronous; allocates. */
+int fpho_issue_calibration(int kind, double *ms, double *wave_instructions,
+                           int *clock_khz);
 int fpho_batch_free(fpho_batch *b);
 
 /* ------------------------------------------ device proposals and gradients */
