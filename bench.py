@@ -79,15 +79,16 @@ def copy_calibration():
 
 def issue_calibration():
     """Wave64 VALU instructions per SIMD per cycle of independent FP32 FMA
-    chains (8-byte and 4-byte encodings), and of FMA chains interleaved with
-    v_exp_f32 (library kernels; the shader clock is the device's nominal
-    one)."""
+    chains (8-byte and 4-byte encodings), of FMA chains interleaved with
+    v_exp_f32, and of v_exp_f32 chains alone (library kernels; the shader
+    clock is the device's nominal one)."""
     import ctypes
     from forcepho_amd import _lib
     _lib.init(0)
     cus = _lib.cu_count()
     out = {}
-    for kind, name in ((0, "fma_f32"), (2, "fmac_f32"), (1, "fma_f32+exp_f32")):
+    for kind, name in ((0, "fma_f32"), (2, "fmac_f32"), (1, "fma_f32+exp_f32"),
+                       (3, "exp_f32")):
         ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _lib.check(_lib.lib().fpho_issue_calibration(kind, ctypes.byref(ms), ctypes.byref(wi),
                                                      ctypes.byref(clk)))
@@ -99,6 +100,18 @@ def issue_calibration():
             # FP32 FMA = 2 FLOP per lane
             out[name]["tflops"] = wi.value * 64 * 2 / (ms.value * 1e-3) / 1e12
     print(json.dumps({"issue_calibration": out, "cus": cus}), flush=True)
+
+
+def exp_issue_rate():
+    """Lane v_exp_f32 evaluations per second of the device, from 16
+    independent exp chains per lane at 8 waves per SIMD on every CU (the
+    library's measurement-only calibration kernel, kind 3)."""
+    import ctypes
+    from forcepho_amd import _lib
+    ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _lib.check(_lib.lib().fpho_issue_calibration(3, ctypes.byref(ms), ctypes.byref(wi),
+                                                 ctypes.byref(clk)))
+    return wi.value * 64 / (ms.value * 1e-3)
 
 
 def measured_traffic(launch):
@@ -367,6 +380,14 @@ def main():
             log(f"cpu reference: {c['pairs_per_s']:.3e} pairs/s on {c['patches']} patches")
 
     traffic, traffic_src = measured_traffic(launch)
+    # exp-issue ceiling: one exp per in-cutoff pair (the algorithmic count)
+    # at the device's measured exp throughput, as a time per sweep
+    exp_rate = exp_issue_rate()
+    exp_ceiling = {"exp_per_s": exp_rate, "ms_per_sweep": n_in / exp_rate * 1e3,
+                   "frac_of_pixel_pass": n_in / exp_rate * 1e3 / eval_ms,
+                   "measured_by": "16 v_exp_f32 chains per lane, 8 waves/SIMD, all CUs"}
+    log(f"exp issue: {exp_rate:.3e} exp/s; one exp per in-cutoff pair takes "
+        f"{exp_ceiling['ms_per_sweep']:.2f} ms of the {eval_ms:.2f} ms pixel pass")
     out = {
         "metric": METRIC, "value": evals_per_s, "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_batched * 1e3,
@@ -383,6 +404,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_source": traffic_src,
+                     "hbm_GBps": (traffic / (eval_ms * 1e-3) / 1e9) if traffic else None,
+                     "exp_issue_ceiling": exp_ceiling,
                      "kernel": "eval pixel pass (eval_kernel / eval_persistent_kernel)",
                      "algorithmic_flop_per_sweep": flops_alg,
                      "pairs_in_cutoff": n_in,

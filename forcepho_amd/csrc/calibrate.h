@@ -5,7 +5,9 @@
 
 namespace fpho {
 // kind 0: 16 FP32 FMA chains per lane; kind 1: 8 FMA and 8 v_exp_f32 chains
-// per lane; kind 2: 16 FMA chains in the accumulate (v_fmac) form.  256 threads per workgroup; writes blocks*256 floats to out.
+// per lane; kind 2: 16 FMA chains in the accumulate (v_fmac) form; kind 3:
+// 16 v_exp_f32 chains per lane.  256 threads per workgroup; writes
+// blocks*256 floats to out.
 // *instr_per_wave = VALU instructions of the measured loop per wave.
 hipError_t launch_issue_calibration(int kind, int blocks, float *out, hipStream_t stream,
                                     int64_t *instr_per_wave);

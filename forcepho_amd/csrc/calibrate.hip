@@ -4,7 +4,8 @@
 // device at hand, how many wave64 VALU instructions a SIMD issues per cycle
 // for the two instruction types that dominate it: FP32 FMA (non-packed, as
 // the build compiles with packed FP32 disabled) and FMA interleaved with
-// v_exp_f32, and FMA in its 4-byte accumulate encoding.  Each lane runs
+// v_exp_f32, FMA in its 4-byte accumulate encoding, and v_exp_f32 alone
+// (the exp-issue ceiling of the pixel pass).  Each lane runs
 // independent dependency chains so that issue,
 // not latency, bounds the rate; 8 waves per SIMD, as the pixel pass runs.
 #include <hip/hip_runtime.h>
@@ -71,6 +72,23 @@ void fmac_chains_kernel(float *__restrict__ out, float k, float b) {
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// 16 independent v_exp_f32 chains per lane (the negation folds into the
+// next exp's source modifier)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void exp_chains_kernel(float *__restrict__ out) {
+    float w[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) w[j] = -1.0f + threadIdx.x * 1e-4f - j * 1e-2f;
+    for (int i = 0; i < kCalIters / 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) w[j] = -__builtin_amdgcn_exp2f(w[j]);  // stays in [-1, 0)
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s += w[j];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 hipError_t launch_issue_calibration(int kind, int blocks, float *out, hipStream_t stream,
                                     int64_t *instr_per_wave) {
     if (kind == 0) {
@@ -81,6 +99,9 @@ hipError_t launch_issue_calibration(int kind, int blocks, float *out, hipStream_
         hipLaunchKernelGGL(fmac_chains_kernel, dim3(blocks), dim3(256), 0, stream, out,
                            1e-7f, 0.001f);
         *instr_per_wave = 16LL * kCalIters;
+    } else if (kind == 3) {
+        hipLaunchKernelGGL(exp_chains_kernel, dim3(blocks), dim3(256), 0, stream, out);
+        *instr_per_wave = 16LL * (kCalIters / 4);
     } else {
         hipLaunchKernelGGL(fma_exp_kernel, dim3(blocks), dim3(256), 0, stream, out, 0.999f,
                            0.001f);
