@@ -3,10 +3,10 @@
 // HBM layout (all structure-of-arrays, per patch unless noted):
 //   pixels   data, ierr, xpix, ypix, residual : float[npix]  (superpixel order,
 //            exposures concatenated, padding pixels carry ierr = 0)
-//   ImgGauss workspace : one 96-byte record per (exposure, source, psf comp)
+//   ImgGauss workspace : one 128-byte record per (exposure, source, psf comp)
 //            written by the convert kernel and read by the pixel kernel with
-//            wave-uniform (scalar) loads; the first 32 bytes are all the
-//            residual pass needs.
+//            wave-uniform (scalar) loads; bytes 16-47 are all the residual
+//            pass needs.
 //   ExpDesc  : one per exposure (batch-global), 128 bytes.
 //   TileDesc : one per wave tile of 64*K pixels of one exposure.
 //   partials : float[n_src*7] per tile + double chi2 per tile; summed in a
