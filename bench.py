@@ -88,7 +88,8 @@ def issue_calibration():
     cus = _lib.cu_count()
     out = {}
     for kind, name in ((0, "fma_f32"), (2, "fmac_f32"), (1, "fma_f32+exp_f32"),
-                       (3, "exp_f32")):
+                       (3, "exp_f32"), (4, "mul_f32"), (5, "add_f32"),
+                       (6, "fmac_f32_vgpr_operands")):
         ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _lib.check(_lib.lib().fpho_issue_calibration(kind, ctypes.byref(ms), ctypes.byref(wi),
                                                      ctypes.byref(clk)))
@@ -96,9 +97,10 @@ def issue_calibration():
         out[name] = {"ms": ms.value, "wave_instructions": wi.value,
                      "clock_mhz": clk.value / 1e3,
                      "per_simd_per_cycle": wi.value / (4 * cus * cycles)}
-        if kind in (0, 2):
-            # FP32 FMA = 2 FLOP per lane
-            out[name]["tflops"] = wi.value * 64 * 2 / (ms.value * 1e-3) / 1e12
+        if kind in (0, 2, 4, 5, 6):
+            # FP32 FMA = 2 FLOP per lane, multiply or add 1
+            flop = 2 if kind in (0, 2, 6) else 1
+            out[name]["tflops"] = wi.value * 64 * flop / (ms.value * 1e-3) / 1e12
     print(json.dumps({"issue_calibration": out, "cus": cus}), flush=True)
 
 

@@ -824,7 +824,7 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
 
 int fpho_issue_calibration(int kind, double *ms, double *wave_instructions, int *clock_khz) {
     if (!ms || !wave_instructions || !clock_khz) return fail(FPHO_E_ARG, "null argument");
-    if (kind < 0 || kind > 3) return fail(FPHO_E_ARG, "kind must be 0 to 3 (got %d)", kind);
+    if (kind < 0 || kind > 6) return fail(FPHO_E_ARG, "kind must be 0 to 6 (got %d)", kind);
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     int cus = 0, clk = 0;

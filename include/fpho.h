@@ -201,8 +201,9 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3);
 /* Measurement only, no counterpart in forcepho: VALU issue calibration of
  * the current device.  kind 0 times independent FP32 FMA chains, kind 1 FMA
  * chains interleaved with v_exp_f32, kind 2 FMA chains in the accumulate
- * (v_fmac) encoding, kind 3 v_exp_f32 chains alone, at 8 waves per SIMD on
- * every CU.
+ * (v_fmac) encoding, kind 3 v_exp_f32 chains alone, kinds 4 and 5 FP32
+ * multiply and add chains, kind 6 FMA chains with vector-register operands
+ * only, at 8 waves per SIMD on every CU.
  * Returns the kernel time (ms, last of three runs), the wave64 VALU
  * instructions it issued and the device's shader clock (kHz), from which
  * instructions per SIMD per cycle follow.  Synchronous; allocates. */
