@@ -245,6 +245,68 @@ def cpu_baseline(patches, specs, thetas, seconds):
                 sample_pairs=pairs, evals=evals)
 
 
+# patch attributes the compiled reference kernel reads (oracle.ReferenceKernel)
+_REF_FIELDS = ("data", "ierr", "xpix", "ypix", "band_start", "band_N", "exposure_start",
+               "exposure_N", "rad2", "D", "crpix", "crval", "CW", "G", "n_psf_per_source",
+               "psfgauss_start", "psfgauss", "n_sources", "n_radii")
+
+
+def _ref_worker(units):
+    """Evaluate (patch fields, proposal) units with the compiled reference
+    kernel in this process; returns the compute seconds."""
+    import types
+    import oracle
+    t = 0.0
+    for fields, prop in units:
+        ref = oracle.ReferenceKernel(types.SimpleNamespace(**fields))
+        t0 = time.perf_counter()
+        ref.evaluate(prop)
+        t += time.perf_counter() - t0
+    return t
+
+
+def cpu_baseline_all_cores(patches, specs, thetas, seconds, workers):
+    """The compiled reference kernel on ``workers`` processes (the pybind11
+    kernel holds the GIL, so threads would not run concurrently): a seeded
+    random sample of patches sized for about ``seconds`` of compute per
+    worker, assigned longest-first to the workers; throughput = sample pairs
+    / the slowest worker's compute time (start-up and data transfer
+    excluded)."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    import oracle
+    if oracle.reference_module() is None:
+        return None
+    rng = np.random.default_rng(4321)
+    order = rng.permutation(len(patches))
+    target = seconds * 1.6e8 * workers
+    sample, pairs = [], 0
+    for i in order:
+        if pairs >= target or len(sample) >= 4 * workers:
+            break
+        sample.append(int(i))
+        pairs += specs[i].n_pairs()
+    sample.sort(key=lambda i: -specs[i].n_pairs())
+    bins = [[] for _ in range(workers)]
+    load = [0] * workers
+    for i in sample:
+        w = int(np.argmin(load))
+        patch, scene, _ = patches[i]
+        fields = {}
+        for name in _REF_FIELDS:
+            fields[name] = (patch.retrieve_array("data") if name == "data"
+                            else getattr(patch, name))
+        scene.set_all_source_params(thetas[i])
+        bins[w].append((fields, scene.get_proposal().copy()))
+        load[w] += specs[i].n_pairs()
+    ctx = multiprocessing.get_context("spawn")
+    with ProcessPoolExecutor(workers, mp_context=ctx) as ex:
+        list(ex.map(_ref_worker, [[] for _ in range(workers)]))   # start every worker
+        times = list(ex.map(_ref_worker, [b for b in bins if b]))
+    return dict(pairs_per_s=pairs / max(times), seconds=max(times), patches=len(sample),
+                sample_pairs=pairs, workers=workers)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -367,7 +429,7 @@ def main():
         plain = len(patches) / t_plain
         log(f"plain: {t_plain * 1e3:.1f} ms/sweep, {plain:.1f} evals/s")
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         c = cpu_baseline(patches, specs, thetas, args.cpu_seconds)
         if c is not None:
@@ -380,6 +442,19 @@ def main():
                               f"{c['seconds']:.1f}s), scaled per pair to the full sweep"),
                    "pairs_per_s": c["pairs_per_s"]}
             log(f"cpu reference: {c['pairs_per_s']:.3e} pairs/s on {c['patches']} patches")
+        workers = min(16, len(os.sched_getaffinity(0)))
+        c = cpu_baseline_all_cores(patches, specs, thetas, args.cpu_seconds / 4, workers)
+        if c is not None:
+            t_sweep = pairs / c["pairs_per_s"]
+            cpu_all = {"value": len(patches) / t_sweep, "unit": "evals/s", "cores": workers,
+                       "kind": "reference",
+                       "sample": (f"forcepho's compute_gaussians_kernel.cc compiled from "
+                                  f"source (g++ -O3), {workers} processes, {c['patches']} "
+                                  f"randomly chosen patches ({c['sample_pairs']:.3e} pairs) "
+                                  f"assigned longest-first, slowest process "
+                                  f"{c['seconds']:.1f}s, scaled per pair to the full sweep"),
+                       "pairs_per_s": c["pairs_per_s"]}
+            log(f"cpu reference, {workers} processes: {c['pairs_per_s']:.3e} pairs/s")
 
     traffic, traffic_src = measured_traffic(launch)
     # exp-issue ceiling: one exp per in-cutoff pair (the algorithmic count)
@@ -416,6 +491,7 @@ def main():
                               "uses no packed FP32); algorithmic FLOP = 98 per pair inside "
                               "the cutoff + 11 per other pair + 7 N_pix n_src + 6 N_pix")},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out), flush=True)
 

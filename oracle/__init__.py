@@ -324,19 +324,13 @@ class ReferenceKernel:
         self.struct = np.array([st], dtype=REF_PATCH_DTYPE)
         self.ptr = int(self.struct.ctypes.data)
 
-    def evaluate(self, proposal, threads=1):
+    def evaluate(self, proposal):
+        """(chi2 [n_bands], dchi2 [n_bands, n_sources, 7]), bands in turn (the
+        pybind11 kernel holds the GIL, so bands cannot run on threads)."""
         p = np.ascontiguousarray(proposal).copy()
         chi2 = np.zeros(self.nb)
         dchi2 = np.zeros((self.nb, self.ns, 7))
-        if threads > 1 and self.nb > 1:
-            from concurrent.futures import ThreadPoolExecutor
-            with ThreadPoolExecutor(threads) as ex:
-                rets = list(ex.map(lambda b: self.mod.EvaluateProposal(b, self.ptr,
-                                                                       p.ctypes.data),
-                                   range(self.nb)))
-        else:
-            rets = [self.mod.EvaluateProposal(b, self.ptr, p.ctypes.data)
-                    for b in range(self.nb)]
+        rets = [self.mod.EvaluateProposal(b, self.ptr, p.ctypes.data) for b in range(self.nb)]
         for b, r in enumerate(rets):
             chi2[b] = r.chi2
             dchi2[b] = np.asarray(r.dchi2_dp).reshape(30, 7)[:self.ns]
