@@ -123,6 +123,15 @@ def lib():
                 f"{LIBPATH} is missing: build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'` "
                 "(forcepho_amd has no CPU fallback)")
+        # torch's wheel carries its own HIP runtime under the same soname
+        # (libamdhip64.so.7). With torch loaded first, this library binds to
+        # that copy and the process has one HIP runtime; loaded first, it
+        # would bring the system copy in and torch's own copy, loaded later,
+        # then finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIBPATH)
         _declare(handle)
         _LIB = handle

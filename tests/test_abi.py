@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -130,3 +131,17 @@ def test_launch_validation():
     l = _lib.Launch(3, 4, 0, 8)
     rc = _lib.lib().fpho_patch_set_launch(ctypes.c_void_p(0), ctypes.byref(l))
     assert rc == 1  # null patch is rejected first
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """torch's wheel carries its own HIP runtime under the system soname;
+    loading the library before torch must not put a second runtime in the
+    process (torch would then find no GPU)."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from forcepho_amd import _lib; _lib.lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n" % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         check=True).stdout.split()
+    assert out[-1] == "1"
