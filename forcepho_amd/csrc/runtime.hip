@@ -79,6 +79,31 @@ struct DevVec {
     }
 };
 
+// Page-locked host buffer (hipHostMalloc): copies from and to it run
+// asynchronously, without a staging copy inside the runtime.
+template <typename T>
+struct PinnedVec {
+    T *ptr = nullptr;
+    int64_t n = 0;
+    PinnedVec() = default;
+    PinnedVec(const PinnedVec &) = delete;
+    PinnedVec &operator=(const PinnedVec &) = delete;
+    ~PinnedVec() { release(); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(int64_t count) {
+        release();
+        if (count <= 0) return hipSuccess;
+        hipError_t e = hipHostMalloc((void **)&ptr, sizeof(T) * (size_t)count,
+                                     hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
 bool finite_all(const float *p, int64_t n) {
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i])) return false;
@@ -178,10 +203,13 @@ struct fpho_patch {
     int precision = kPrecMixed;
     Plan plan;
     uint64_t plan_generation = 0;
-    // staging for the synchronous drop-in
+    // staging for the synchronous drop-in: the proposal, and chi2 [n_bands]
+    // followed by dchi2 [n_bands][n_sources][7] in one buffer, so a call
+    // makes one copy each way from page-locked memory and one wait
     DevVec<fpho_source> prop_dev;
-    DevVec<double> chi2_dev, dchi2_dev;
-    std::vector<double> dchi2_host;
+    DevVec<double> out_dev;
+    PinnedVec<fpho_source> prop_host;
+    PinnedVec<double> out_host;
 };
 
 struct fpho_batch {
@@ -709,19 +737,22 @@ int fpho_evaluate(fpho_patch *p, const fpho_source *prop, double *chi2, float *d
     int rc = ensure_patch_plan(p);
     if (rc) return rc;
     const int64_t nd = (int64_t)p->n_bands * p->n_sources * kNParams;
+    const int64_t nout = p->n_bands + nd;
     if (p->prop_dev.n < p->n_sources) HIP_TRY(p->prop_dev.alloc(p->n_sources));
-    if (p->chi2_dev.n < p->n_bands) HIP_TRY(p->chi2_dev.alloc(p->n_bands));
-    if (p->dchi2_dev.n < nd) HIP_TRY(p->dchi2_dev.alloc(nd));
-    HIP_TRY(hipMemcpy(p->prop_dev.ptr, prop, sizeof(fpho_source) * p->n_sources,
-                      hipMemcpyHostToDevice));
-    rc = run_plan(p->plan, p->prop_dev.ptr, p->chi2_dev.ptr, p->dchi2_dev.ptr, nullptr);
+    if (p->out_dev.n < nout) HIP_TRY(p->out_dev.alloc(nout));
+    if (p->prop_host.n < p->n_sources) HIP_TRY(p->prop_host.alloc(p->n_sources));
+    if (p->out_host.n < nout) HIP_TRY(p->out_host.alloc(nout));
+    std::memcpy(p->prop_host.ptr, prop, sizeof(fpho_source) * p->n_sources);
+    HIP_TRY(hipMemcpyAsync(p->prop_dev.ptr, p->prop_host.ptr,
+                           sizeof(fpho_source) * p->n_sources, hipMemcpyHostToDevice, nullptr));
+    rc = run_plan(p->plan, p->prop_dev.ptr, p->out_dev.ptr, p->out_dev.ptr + p->n_bands,
+                  nullptr);
     if (rc) return rc;
-    p->dchi2_host.resize(nd);
-    HIP_TRY(hipMemcpy(chi2, p->chi2_dev.ptr, sizeof(double) * p->n_bands,
-                      hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(p->dchi2_host.data(), p->dchi2_dev.ptr, sizeof(double) * nd,
-                      hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < nd; ++i) dchi2[i] = (float)p->dchi2_host[i];
+    HIP_TRY(hipMemcpyAsync(p->out_host.ptr, p->out_dev.ptr, sizeof(double) * nout,
+                           hipMemcpyDeviceToHost, nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    std::memcpy(chi2, p->out_host.ptr, sizeof(double) * p->n_bands);
+    for (int64_t i = 0; i < nd; ++i) dchi2[i] = (float)p->out_host.ptr[p->n_bands + i];
     return FPHO_OK;
 }
 
