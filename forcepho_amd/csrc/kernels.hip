@@ -638,21 +638,31 @@ __global__ void zero_counter_kernel(int32_t *counter) {
 // ------------------------------------------------------------------ reduce
 
 // One workgroup per (patch, band).  Column j of the band's [ntiles][n_src*7]
-// partial block is summed by one lane in tile order; chi2 partials by a
-// fixed-shape tree.  The result does not depend on tile execution order.
+// partial block is cut into S segments of consecutive tiles, S a function of
+// the column count only (S * ncol <= kReduceSlots); each segment is summed in
+// tile order by one lane, then each column's S segment sums in segment
+// order.  chi2 partials go through a fixed-shape tree.  The result depends
+// only on (ntiles, ncol), not on tile execution order or the launch.
+constexpr int kReduceSlots = 4096;
+
 template <typename Acc>
 __global__ __launch_bounds__(256) void reduce_kernel(
     const OutDesc *__restrict__ outs, const float *__restrict__ gpart,
     const double *__restrict__ cpart, double *__restrict__ chi2,
     double *__restrict__ dchi2) {
+    __shared__ Acc seg[kReduceSlots];
     __shared__ Acc sh[256];
     const OutDesc o = outs[blockIdx.x];
     const int ncol = o.n_src * kNParams;
-    for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+    const int S = max(1, min(16, kReduceSlots / ncol));
+    const int len = (o.ntiles + S - 1) / S;   // tiles per segment
+    for (int w = threadIdx.x; w < ncol * S; w += blockDim.x) {
+        const int j = w % ncol, r = w / ncol;
+        const int t0 = r * len, t1 = min(o.ntiles, t0 + len);
         const float *col = gpart + o.part0 + j;
         Acc acc = 0;
-        int t = 0;
-        for (; t + 4 <= o.ntiles; t += 4) {
+        int t = t0;
+        for (; t + 4 <= t1; t += 4) {
             const float v0 = col[(int64_t)(t + 0) * ncol];
             const float v1 = col[(int64_t)(t + 1) * ncol];
             const float v2 = col[(int64_t)(t + 2) * ncol];
@@ -662,13 +672,18 @@ __global__ __launch_bounds__(256) void reduce_kernel(
             acc += (Acc)v2;
             acc += (Acc)v3;
         }
-        for (; t < o.ntiles; ++t) acc += (Acc)col[(int64_t)t * ncol];
-        dchi2[o.dchi2_out + j] = (double)acc;
+        for (; t < t1; ++t) acc += (Acc)col[(int64_t)t * ncol];
+        seg[r * ncol + j] = acc;
     }
     Acc c = 0;
     for (int t = threadIdx.x; t < o.ntiles; t += blockDim.x) c += (Acc)cpart[o.tile0 + t];
     sh[threadIdx.x] = c;
     __syncthreads();
+    for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+        Acc acc = 0;
+        for (int r = 0; r < S; ++r) acc += seg[r * ncol + j];
+        dchi2[o.dchi2_out + j] = (double)acc;
+    }
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
