@@ -319,9 +319,9 @@ def main():
         run_config5(args)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not "
-                          "part of it (patches would be sharded across GPUs, see "
-                          "DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "runs on one GPU only: more GPUs would each take a "
+                          "share of the patches (independent units, balanced by pair "
+                          "count), which is not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
