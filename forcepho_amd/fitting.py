@@ -1,4 +1,10 @@
-"""Linear flux optimisation on the device.
+"""Fitting drivers around the posterior: optimisers, NUTS and the linear
+flux solve.
+
+``run_opt`` / ``run_opt_bounded`` (scipy BFGS / L-BFGS-B) and ``run_lmc``
+(NUTS) keep the interfaces of forcepho/fitting.py:255-499 and drive any
+posterior object with ``lnprob_and_grad`` / ``nll`` and ``transform``, such
+as :class:`forcepho_amd.model.FastPosterior` over the HIP proposer.
 
 ``optimize_fluxes`` is forcepho's conditional maximum-likelihood flux solve
 (forcepho/fitting.py:573-627): given fixed source shapes, the model is linear
@@ -10,10 +16,130 @@ precision matrix.  The normal equations of every band are formed and solved
 in FP64 on the GPU in one batched call.
 """
 
+import time
+
 import numpy as np
 import torch
 
-__all__ = ["optimize_one_band", "optimize_fluxes", "solve_normal_equations"]
+__all__ = ["Result", "run_opt", "run_opt_bounded", "run_lmc", "optimize_one_band",
+           "optimize_fluxes", "solve_normal_equations"]
+
+
+class Result:
+    """A namespace for the products of a fit (the in-memory part of
+    forcepho/fitting.py:24-35; the HDF5 file round trip is not built)."""
+
+    def __init__(self, **kwargs):
+        for k, v in kwargs.items():
+            setattr(self, k, v)
+
+
+def _start(model, q):
+    q = np.asarray(q, dtype=np.float64)
+    return model.transform.inverse_transform(q) if model.transform is not None else q.copy()
+
+
+def _optimize(model, q, method, jac, callback, bounds, opts):
+    from scipy.optimize import minimize
+    start = _start(model, q)
+    t0 = time.time()
+    fun = model.nll if jac else model.nll_nograd
+    scires = minimize(fun, start.copy(), jac=True if jac else None, method=method,
+                      options=opts, bounds=bounds, callback=callback)
+    chain = np.atleast_2d(scires.x)
+    if model.transform is not None:
+        chain = model.transform.transform(chain)
+    result = Result(ndim=len(q), starting_position=np.array(q, dtype=np.float64).copy(),
+                    chain=chain, lnp=-0.5 * scires.fun, wall_time=time.time() - t0,
+                    fun=scires.fun, jac=scires.jac, message=scires.message)
+    return result, scires
+
+
+def run_opt(model, q, jac=True, callback=None, **extras):
+    """BFGS on the negative log posterior in the sampling space (the
+    interface of forcepho/fitting.py:415-454): ``model`` provides ``nll``
+    ((value, gradient) for ``jac``) or ``nll_nograd`` and ``transform``; ``q``
+    is the start in the constrained space.  Returns (Result, scipy result);
+    ``Result.chain`` holds the optimum in the constrained space."""
+    return _optimize(model, q, "BFGS", jac, callback, None, dict(extras))
+
+
+def run_opt_bounded(model, q, jac=True, callback=None, **extras):
+    """L-BFGS-B (forcepho/fitting.py:457-499): ``bounds`` in ``extras``,
+    replaced by (-200, 200) per parameter when the model samples in a
+    transformed space."""
+    bounds = extras.pop("bounds", None)
+    opts = {"ftol": 1e-5, "gtol": 1e-5, "maxcor": 20}
+    opts.update(extras)
+    if model.transform is not None:
+        bounds = [(-200.0, 200.0)] * len(q)
+    return _optimize(model, q, "L-BFGS-B", jac, callback, bounds, opts)
+
+
+class _ModelPosterior:
+    """One chain over a host posterior object (``lnprob_and_grad`` of the
+    sampling-space vector, as forcepho's Posterior), in the form the device
+    samplers use: lnp [1] and gradient [ndim] tensors."""
+
+    def __init__(self, model, ndim, device):
+        self.model = model
+        self.lower = torch.zeros(1, dtype=torch.float64, device=device)
+        self.seg_idx = torch.arange(ndim, device=device)[None, :]
+        self.seg_mask = torch.ones((1, ndim), dtype=torch.float64, device=device)
+        self.param_patch = torch.zeros(ndim, dtype=torch.long, device=device)
+
+    def segment_sum(self, v):
+        return v.sum().reshape(1)
+
+    def __call__(self, z):
+        lnp, grad = self.model.lnprob_and_grad(z.detach().cpu().numpy())
+        dev = self.lower.device
+        return (torch.tensor([float(lnp)], dtype=torch.float64, device=dev),
+                torch.as_tensor(np.asarray(grad, dtype=np.float64), device=dev))
+
+
+def run_lmc(model, q, n_draws, adapt=False, full=False, weight=10, warmup=[10], trace=None,
+            z_cov=None, max_treedepth=10, discard_tuned_samples=True, progressbar=False,
+            random_seed=0xDEADBEEF):
+    """NUTS draws from ``model`` (the interface of forcepho/fitting.py:255-342,
+    which drives littlemcmc): multinomial NUTS of :mod:`forcepho_amd.nuts` on
+    ``model.lnprob_and_grad``, ``warmup[0]`` warm-up iterations adapting the
+    step size and a diagonal (or with ``full`` a dense) mass matrix, started
+    from ``z_cov`` when given.  Warm-up states are never returned.  ``adapt``,
+    ``weight``, ``trace``, ``discard_tuned_samples`` and ``progressbar``
+    belong to littlemcmc and are accepted for compatibility.  Returns
+    (Result with ``chain`` [n_draws, ndim] in the constrained space, the
+    sampler, per-draw statistics)."""
+    from .nuts import NUTS
+    start = _start(model, q)
+    ndim = len(start)
+    dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+    post = _ModelPosterior(model, ndim, dev)
+    inv_mass = None
+    if z_cov is not None:
+        z_cov = np.asarray(z_cov, dtype=np.float64)
+        inv_mass = z_cov[None] if full else np.diag(z_cov)
+    t0 = time.time()
+    sampler = NUTS(post, start, step_size=0.1, max_treedepth=max_treedepth,
+                   seed=int(random_seed) % (2 ** 31), inv_mass=inv_mass, dense=full)
+    n_tune = int(warmup[0]) if len(warmup) else 0
+    if n_tune > 0:
+        sampler.adapt(n_tune)
+    draws, stats = [], []
+    for _ in range(int(n_draws)):
+        sampler.trajectory()
+        draws.append(sampler.z.cpu().numpy().copy())
+        stats.append({"tree_depth": int(sampler.tree_depth[0]),
+                      "n_steps": int(sampler.n_leapfrog[0]),
+                      "diverging": bool(sampler.divergent[0]),
+                      "mean_tree_accept": float(sampler.accept_prob[0]),
+                      "step_size": float(sampler.eps_patch[0])})
+    z = np.array(draws)
+    chain = model.transform.transform(z) if model.transform is not None else z
+    result = Result(ndim=len(q), starting_position=np.array(q, dtype=np.float64).copy(),
+                    chain=chain, n_call=getattr(model, "ncall", None),
+                    wall_time=time.time() - t0)
+    return result, sampler, stats
 
 
 def _device():
