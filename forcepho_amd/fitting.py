@@ -21,7 +21,8 @@ import time
 import numpy as np
 import torch
 
-__all__ = ["Result", "run_opt", "run_opt_bounded", "run_lmc", "optimize_one_band",
+__all__ = ["Result", "run_opt", "run_opt_bounded", "run_lmc", "initial_cov",
+           "optimize_one_band",
            "optimize_fluxes", "solve_normal_equations"]
 
 
@@ -98,15 +99,38 @@ class _ModelPosterior:
                 torch.as_tensor(np.asarray(grad, dtype=np.float64), device=dev))
 
 
+def initial_cov(n_dim, init_cov=None, trace=None, regular_variance=1e-3):
+    """The covariance ``get_pot`` starts the mass matrix from
+    (forcepho/fitting.py:370-412): ``init_cov``, else the covariance of the
+    samples in ``trace`` ([n_dim, n_samples]), else the identity; its
+    diagonal is raised by ``regular_variance`` (up to 5 times) until it has
+    a Cholesky factor."""
+    if init_cov is not None:
+        cov = np.array(init_cov, dtype=np.float64)
+    elif trace is not None:
+        cov = np.atleast_2d(np.cov(np.asarray(trace, dtype=np.float64), rowvar=True))
+    else:
+        cov = np.eye(n_dim)
+    for _ in range(5):
+        try:
+            np.linalg.cholesky(cov)
+            break
+        except np.linalg.LinAlgError:
+            cov[np.diag_indices_from(cov)] += regular_variance
+    return cov
+
+
 def run_lmc(model, q, n_draws, adapt=False, full=False, weight=10, warmup=[10], trace=None,
             z_cov=None, max_treedepth=10, discard_tuned_samples=True, progressbar=False,
             random_seed=0xDEADBEEF):
     """NUTS draws from ``model`` (the interface of forcepho/fitting.py:255-342,
     which drives littlemcmc): multinomial NUTS of :mod:`forcepho_amd.nuts` on
     ``model.lnprob_and_grad``, ``warmup[0]`` warm-up iterations adapting the
-    step size and a diagonal (or with ``full`` a dense) mass matrix, started
-    from ``z_cov`` when given.  Warm-up states are never returned.  ``adapt``,
-    ``weight``, ``trace``, ``discard_tuned_samples`` and ``progressbar``
+    step size and a diagonal (or with ``full`` a dense) inverse mass matrix,
+    started from ``z_cov``, else from the covariance of ``trace`` ([ndim,
+    n_samples]), else the identity, as ``get_pot`` chooses it
+    (forcepho/fitting.py:370-412).  Warm-up states are never returned.
+    ``adapt``, ``weight``, ``discard_tuned_samples`` and ``progressbar``
     belong to littlemcmc and are accepted for compatibility.  Returns
     (Result with ``chain`` [n_draws, ndim] in the constrained space, the
     sampler, per-draw statistics)."""
@@ -116,9 +140,9 @@ def run_lmc(model, q, n_draws, adapt=False, full=False, weight=10, warmup=[10], 
     dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
     post = _ModelPosterior(model, ndim, dev)
     inv_mass = None
-    if z_cov is not None:
-        z_cov = np.asarray(z_cov, dtype=np.float64)
-        inv_mass = z_cov[None] if full else np.diag(z_cov)
+    cov = initial_cov(ndim, z_cov, trace)
+    if z_cov is not None or trace is not None:
+        inv_mass = cov[None] if full else np.diag(cov).copy()
     t0 = time.time()
     sampler = NUTS(post, start, step_size=0.1, max_treedepth=max_treedepth,
                    seed=int(random_seed) % (2 ** 31), inv_mass=inv_mass, dense=full)
