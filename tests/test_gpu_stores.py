@@ -52,3 +52,31 @@ def test_simplepatch_lnlike_and_grad(hip, survey):
     p.build_patch(CircularRegion(CENTER[0], CENTER[1], 0.8 / 3600), survey["catalog"],
                   allbands=ALLBANDS)
     _check(p, "SimplePatch")
+
+
+def test_prepare_model_and_fit(hip, survey):
+    """The user workflow of forcepho: pack a region from FITS files, wrap it
+    in a posterior with ``prepare_model`` (no fixed sources, no bounds), and
+    its lnp and gradient at the catalog match the FP64 oracle; BFGS from the
+    catalog does not lose posterior probability."""
+    from forcepho_amd.fitting import run_opt
+    p = SimplePatch(fitsfiles=survey["paths"]["fits"], psfstore=survey["paths"]["psf"])
+    p.build_patch(CircularRegion(CENTER[0], CENTER[1], 0.8 / 3600), survey["catalog"],
+                  allbands=ALLBANDS)
+    try:
+        model, q = p.prepare_model(active=survey["catalog"])
+        lnp, grad = model.lnprob_and_grad(q)
+        prop = proposal_for(model.scene, q)
+        c_ref, d_ref = oracle.evaluate(p, prop)
+        ll_ref = -0.5 * c_ref.sum()
+        g_ref = -0.5 * oracle.stack_grad(d_ref)
+        print(f"lnp {lnp:.6f} oracle {ll_ref:.6f}", flush=True)
+        assert abs(lnp - ll_ref) <= CHI2_RTOL * abs(ll_ref)
+        err = np.abs(grad - g_ref)
+        assert np.all((err <= 1e-4 * np.abs(g_ref)) | (err <= 1e-6 * np.abs(g_ref).max()))
+        res, sci = run_opt(model, q, maxiter=20)
+        lnp_opt = model.lnprob(res.chain[0])
+        print(f"BFGS: lnp {lnp:.3f} -> {lnp_opt:.3f} in {sci.nit} iterations", flush=True)
+        assert lnp_opt >= lnp - 1e-6 * abs(lnp)
+    finally:
+        p.free()
