@@ -644,14 +644,15 @@ __global__ void zero_counter_kernel(int32_t *counter) {
 // order.  chi2 partials go through a fixed-shape tree.  The result depends
 // only on (ntiles, ncol), not on tile execution order or the launch.
 constexpr int kReduceSlots = 4096;
+constexpr int kReduceThreads = 1024;   // a single patch has few (patch, band) groups
 
 template <typename Acc>
-__global__ __launch_bounds__(256) void reduce_kernel(
+__global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     const OutDesc *__restrict__ outs, const float *__restrict__ gpart,
     const double *__restrict__ cpart, double *__restrict__ chi2,
     double *__restrict__ dchi2) {
     __shared__ Acc seg[kReduceSlots];
-    __shared__ Acc sh[256];
+    __shared__ Acc sh[kReduceThreads];
     const OutDesc o = outs[blockIdx.x];
     const int ncol = o.n_src * kNParams;
     const int S = max(1, min(16, kReduceSlots / ncol));
@@ -917,11 +918,11 @@ hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart
                          hipStream_t stream) {
     if (n_outs <= 0) return hipSuccess;
     if (fp64)
-        hipLaunchKernelGGL(reduce_kernel<double>, dim3((unsigned)n_outs), dim3(256), 0, stream,
-                           outs, gpart, cpart, chi2, dchi2);
+        hipLaunchKernelGGL(reduce_kernel<double>, dim3((unsigned)n_outs), dim3(kReduceThreads), 0,
+                           stream, outs, gpart, cpart, chi2, dchi2);
     else
-        hipLaunchKernelGGL(reduce_kernel<float>, dim3((unsigned)n_outs), dim3(256), 0, stream,
-                           outs, gpart, cpart, chi2, dchi2);
+        hipLaunchKernelGGL(reduce_kernel<float>, dim3((unsigned)n_outs), dim3(kReduceThreads), 0,
+                           stream, outs, gpart, cpart, chi2, dchi2);
     return hipGetLastError();
 }
 
