@@ -169,9 +169,9 @@ int fpho_evaluate_async(fpho_patch *p, const fpho_source *proposal_dev,
                         double *chi2_dev, double *dchi2_dev, void *stream);
 
 /* ------------------------------------------------------------------ batch */
-/* Many patches evaluated by one set of launches: convert, one persistent
- * pixel kernel over (patch, band, exposure, pixel-block) tiles sorted by
- * cost, one fixed-order FP64 reduction.  Patches stay owned by the caller and
+/* Many patches evaluated by one set of launches: convert, one pixel kernel
+ * (a grid launch or a persistent queue) over (patch, band, exposure,
+ * pixel-block) tiles sorted by cost, one fixed-order FP64 reduction.  Patches stay owned by the caller and
  * must outlive the batch. */
 int fpho_batch_create(fpho_patch *const *patches, int n_patches,
                       fpho_batch **out);
