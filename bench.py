@@ -2,13 +2,18 @@
 """Benchmark: lnlike+grad evaluations/s on configuration 4 (512 uneven
 patches: sources 1+Poisson(8) capped at 30, bands U{1..10}, exposures per
 band U{1..16}, square tiles of side U{32..256}, 3-6 PSF Gaussians per band),
-batched (one persistent launch over all patches, parameters -> proposals ->
-image Gaussians -> pixel pass -> FP64 reduction -> stacked gradient, all on
-the device) versus the plain single-patch drop-in (one synchronous
-``HIPProposer.evaluate_proposal`` per patch).
+batched (one launch of each stage over all patches, parameters -> proposals
+-> image Gaussians -> pixel pass -> FP64 reduction -> stacked gradient, all
+on the device, at the autotuned launch shape) versus the plain single-patch
+drop-in (one synchronous ``HIPProposer.evaluate_proposal`` per patch).
 
 One step = one lnlike+gradient evaluation of every patch.  ``value`` is
 patch-evaluations per second of the batched path (inputs resident in HBM).
+The JSON line also carries the roofline of the pixel pass (algorithmic FLOP
+over its HIP-event time; HBM bytes per launch and GB/s from the committed
+byte-counter summary when it was taken on this kernel source; the device's
+measured exp-issue ceiling), the compiled reference CPU kernel on one core
+and on all cores of the host, and the batched/plain ratio.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 
