@@ -81,6 +81,9 @@ def load():
 def _arrays(patch):
     """Contiguous copies of the patch arrays with the device dtypes."""
     a = {}
+    for name in ("data", "ierr", "xpix", "ypix", "psfgauss"):
+        if getattr(patch, name, None) is None:
+            raise ValueError(f"patch.{name} is not on the host")
     for name, dt in [("data", np.float32), ("ierr", np.float32), ("xpix", np.float32),
                      ("ypix", np.float32), ("band_start", np.int16), ("band_N", np.int16),
                      ("exposure_start", np.int32), ("exposure_N", np.int32),
