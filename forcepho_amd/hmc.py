@@ -240,9 +240,10 @@ class HMC:
         mean = torch.zeros_like(self.z)
         m2 = torch.zeros_like(self.z)
         if self.dense is not None:
-            P, m = self.dense.idx.shape
-            cmean = torch.zeros((P, m), dtype=torch.float64, device=self.z.device)
-            cm2 = torch.zeros((P, m, m), dtype=torch.float64, device=self.z.device)
+            n_blocks, width = self.dense.idx.shape
+            cmean = torch.zeros((n_blocks, width), dtype=torch.float64, device=self.z.device)
+            cm2 = torch.zeros((n_blocks, width, width), dtype=torch.float64,
+                              device=self.z.device)
         n_mass = 0
         for it in range(n_warmup):
             self.trajectory()

@@ -59,3 +59,31 @@ def test_divergent_trajectory_is_rejected():
     hmc.trajectory()
     assert hmc.accept_prob.item() < 1e-6
     assert torch.equal(hmc.z, z) or hmc.n_accept.item() == 0 or torch.isfinite(hmc.z).all()
+
+
+class _Recorder(HMC):
+    """HMC that records each patch's step size before every trajectory."""
+
+    def trajectory(self):
+        self.eps_seen.append(self.eps_patch.clone())
+        super().trajectory()
+
+
+def test_dense_and_diagonal_adaptation_share_the_first_window():
+    """With an identity mass both samplers draw the same momenta and take
+    the same steps, so the dual-averaging step sizes must agree bit for bit
+    until the first mass-matrix window ends (the dense warm-up once restarted
+    its step-size schedule at the block width instead of iteration 1)."""
+    theta_off = np.array([0, 3])
+    post = GaussianPatches(np.ones(3), theta_off)
+    z0 = torch.as_tensor([0.3, -0.2, 0.1], dtype=torch.float64)
+    n_warmup, first_window = 200, 25
+    runs = []
+    for dense in (False, True):
+        h = _Recorder(post, z0, step_size=0.05, n_steps=5, seed=7, dense=dense)
+        h.eps_seen = []
+        h.adapt(n_warmup, first_window=first_window)
+        runs.append(torch.stack(h.eps_seen))
+    first_end = int(0.15 * n_warmup) + first_window   # trajectories before the first update
+    assert torch.equal(runs[0][:first_end + 1], runs[1][:first_end + 1])
+    assert not torch.equal(runs[0][1:2], runs[0][0:1])  # the schedule did move
