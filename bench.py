@@ -34,6 +34,7 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
 FLOP_PER_PAIR = 98         # algorithmic FLOPs per (pixel, image Gaussian) pair
 FLOP_PER_CUT_PAIR = 11     # dx, dy, v, exp argument of a pair beyond the cutoff
 TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "pixel_pass_config4_traffic.txt")
+CPU_SHARE = 16             # host cores per GPU on the MI355X nodes this runs on
 
 
 def log(*a):
@@ -50,6 +51,7 @@ def parse():
     ap.add_argument("--plain-sweeps", type=int, default=1)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-hmc", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--launch", type=str, default=None,
@@ -85,9 +87,9 @@ def copy_calibration():
 def issue_calibration():
     """Wave64 VALU instructions per SIMD per cycle of independent FP32 FMA
     chains (8-byte and 4-byte encodings), of FMA chains interleaved with
-    v_exp_f32, and of v_exp_f32 chains alone (library kernels; the shader
-    clock is the device's nominal one)."""
-    import ctypes
+    v_exp_f32, and of v_exp_f32 chains alone (measurement library; the
+    shader clock is the device's nominal one)."""
+    import measure
     from forcepho_amd import _lib
     _lib.init(0)
     cus = _lib.cu_count()
@@ -95,37 +97,47 @@ def issue_calibration():
     for kind, name in ((0, "fma_f32"), (2, "fmac_f32"), (1, "fma_f32+exp_f32"),
                        (3, "exp_f32"), (4, "mul_f32"), (5, "add_f32"),
                        (6, "fmac_f32_vgpr_operands")):
-        ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-        _lib.check(_lib.lib().fpho_issue_calibration(kind, ctypes.byref(ms), ctypes.byref(wi),
-                                                     ctypes.byref(clk)))
-        cycles = ms.value * 1e-3 * clk.value * 1e3
-        out[name] = {"ms": ms.value, "wave_instructions": wi.value,
-                     "clock_mhz": clk.value / 1e3,
-                     "per_simd_per_cycle": wi.value / (4 * cus * cycles)}
+        ms, wi, clk = measure.issue_calibration(kind)
+        cycles = ms * 1e-3 * clk * 1e3
+        out[name] = {"ms": ms, "wave_instructions": wi, "clock_mhz": clk / 1e3,
+                     "per_simd_per_cycle": wi / (4 * cus * cycles)}
         if kind in (0, 2, 4, 5, 6):
             # FP32 FMA = 2 FLOP per lane, multiply or add 1
             flop = 2 if kind in (0, 2, 6) else 1
-            out[name]["tflops"] = wi.value * 64 * flop / (ms.value * 1e-3) / 1e12
+            out[name]["tflops"] = wi * 64 * flop / (ms * 1e-3) / 1e12
     print(json.dumps({"issue_calibration": out, "cus": cus}), flush=True)
 
 
 def exp_issue_rate():
     """Lane v_exp_f32 evaluations per second of the device, from 16
     independent exp chains per lane at 8 waves per SIMD on every CU (the
-    library's measurement-only calibration kernel, kind 3)."""
-    import ctypes
-    from forcepho_amd import _lib
-    ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-    _lib.check(_lib.lib().fpho_issue_calibration(3, ctypes.byref(ms), ctypes.byref(wi),
-                                                 ctypes.byref(clk)))
-    return wi.value * 64 / (ms.value * 1e-3)
+    measurement library's calibration kernel, kind 3)."""
+    import measure
+    ms, wi, clk = measure.issue_calibration(3)
+    return wi * 64 / (ms * 1e-3)
+
+
+def host_cpu():
+    """Model name and logical CPU count of this host, and the CPUs this
+    process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0))}
 
 
 def measured_traffic(launch):
     """HBM bytes per pixel-pass launch from the committed rocprofv3
     byte-counter summary, if it was taken on this kernel source and launch
     shape; else (None, reason)."""
-    from forcepho_amd import profiling
+    import profiling
     if not os.path.exists(TRAFFIC_SUMMARY):
         return None, "no byte-counter summary"
     kv = profiling.read_summary(TRAFFIC_SUMMARY)
@@ -154,9 +166,10 @@ def algorithmic_counts(specs):
 def build_patches(specs, log_every=64, config=4, keep_host=None):
     """Upload every patch with data = noise, render the truth on the device
     with negated fluxes (the model is linear in flux) so that the residual
-    buffer holds noise + model, and make it the data (swap_on_device).
-    Host pixel arrays are dropped after upload except for the patches in
-    ``keep_host`` (None keeps all)."""
+    buffer holds noise + model, and make it the data (swap_on_device);
+    then stop residual writes, so that evaluations write nothing but their
+    outputs.  Host pixel arrays are dropped after upload except for the
+    patches in ``keep_host`` (None keeps all)."""
     import ctypes
     from forcepho_amd import _lib, synth
     from forcepho_amd.limits import source_struct_dtype
@@ -183,6 +196,7 @@ def build_patches(specs, log_every=64, config=4, keep_host=None):
                                             _lib.fptr(c, ctypes.c_double),
                                             _lib.fptr(d, ctypes.c_float)))
         patch.swap_on_device()
+        patch.set_residual_writes(False)
         patch.data = None  # host copy no longer describes the device data
         if keep_host is not None and i not in keep_host:
             patch.ierr = None   # xpix stays: it sizes the patch (npix)
@@ -199,20 +213,42 @@ def time_plain(patch_list, props_host, sweeps=1):
     download: the residual stays on the device)."""
     from forcepho_amd.proposal import HIPProposer
     prop_er = HIPProposer()
-    saved = [p.return_residual for p in patch_list]
-    for p in patch_list:
-        p.return_residual = False
-    try:
-        for p, pr in zip(patch_list, props_host):    # warm-up (plans, staging)
+    for p, pr in zip(patch_list, props_host):    # warm-up (plans, staging)
+        prop_er.evaluate_proposal(pr, patch=p)
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        for p, pr in zip(patch_list, props_host):
             prop_er.evaluate_proposal(pr, patch=p)
-        t0 = time.perf_counter()
-        for _ in range(sweeps):
-            for p, pr in zip(patch_list, props_host):
-                prop_er.evaluate_proposal(pr, patch=p)
-        return (time.perf_counter() - t0) / sweeps
-    finally:
-        for p, r in zip(patch_list, saved):
-            p.return_residual = r
+    return (time.perf_counter() - t0) / sweeps
+
+
+def time_hmc(patches, batch, splines, thetas, n_traj=3, n_steps=10, eps=1e-3):
+    """Graph-captured HMC over every patch of the batch: one trajectory of
+    ``n_steps`` leapfrog steps (each a full lnlike+gradient of every patch
+    through the sampling-space transform) recorded in a HIP graph and
+    replayed ``n_traj`` times."""
+    import torch
+    from forcepho_amd import synth
+    from forcepho_amd.hmc import HMC, DevicePosterior
+    lo, hi = [], []
+    for (p, sc, th) in patches:
+        l, h = synth.theta_bounds(th, len(p.band_N))
+        lo.append(l)
+        hi.append(h)
+    lo, hi = np.concatenate(lo), np.concatenate(hi)
+    post = DevicePosterior(batch, splines, lo, hi)
+    z0 = post.inverse_transform(np.clip(np.concatenate(thetas), lo + 1e-9, hi - 1e-9))
+    hmc = HMC(post, z0, step_size=eps, n_steps=n_steps, seed=5)
+    hmc.capture(warmup=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hmc.run(n_traj)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"evals_per_s": len(patches) * n_steps * n_traj / dt,
+            "ms_per_leapfrog_step": dt / (n_steps * n_traj) * 1e3,
+            "trajectories": n_traj, "leapfrog_steps": n_steps, "step_size": eps,
+            "acceptance": float(hmc.n_accept.mean().item()) / (n_traj + 1)}
 
 
 def cpu_baseline(patches, specs, thetas, seconds):
@@ -395,7 +431,8 @@ def main():
     # The reference algorithm evaluates a pair fully only inside the cutoff
     # (exp argument <= 36, common_kernel.cc:73,108) and stops after the exp
     # argument otherwise: 98 FLOP per in-cutoff pair, 11 per other pair.
-    n_in = batch.count_pairs_in_cutoff()
+    import measure
+    n_in = measure.count_pairs_in_cutoff(batch)
     flops_alg = flops - FLOP_PER_PAIR * pairs + FLOP_PER_PAIR * n_in + \
         FLOP_PER_CUT_PAIR * max(pairs - n_in, 0)
     achieved = flops_alg / (eval_ms * 1e-3) / 1e12
@@ -410,7 +447,7 @@ def main():
         from forcepho_amd.limits import source_struct_dtype
         props = batch.proposals.cpu().numpy().view(source_struct_dtype)
         chi2_dev = batch.chi2_host()
-        small = np.argsort([s.n_pairs() for s in specs])[:3]
+        small = [int(i) for i in np.argsort([s.n_pairs() for s in specs])[:3]]
         errs = []
         for i in small:
             p, sc, th = patches[i]
@@ -419,9 +456,20 @@ def main():
             c_ref, _ = oracle.evaluate(p, prop)
             errs.append(np.max(np.abs(chi2_dev[i] - c_ref) / np.abs(c_ref)))
             p.data = None
-        chi2_err = float(max(errs))
+        chi2_err = {"max_rel_err": float(max(errs)), "patches": small,
+                    "n_pairs": [specs[i].n_pairs() for i in small],
+                    "note": "the three smallest patches, FP64 oracle on the host; the "
+                            "largest, median and most-sourced patches are checked "
+                            "against committed oracle fixtures by the GPU tests"}
     except ImportError:
         pass
+
+    # ---- graph-captured HMC leapfrog over every patch (10 steps/trajectory)
+    hmc_rate = None
+    if not args.skip_hmc:
+        hmc_rate = time_hmc(patches, batch, splines, thetas, n_traj=3)
+        log(f"graph-captured HMC: {hmc_rate['evals_per_s']:.1f} evals/s "
+            f"({hmc_rate['ms_per_leapfrog_step']:.2f} ms per leapfrog step)")
 
     # ---- plain single-patch drop-in
     plain = None
@@ -440,19 +488,22 @@ def main():
         if c is not None:
             t_sweep = pairs / c["pairs_per_s"]
             cpu = {"value": len(patches) / t_sweep, "unit": "evals/s", "cores": 1,
-                   "kind": "reference",
+                   "kind": "reference", "host": host_cpu(),
                    "sample": (f"forcepho's compute_gaussians_kernel.cc compiled from source "
                               f"(g++ -O3), 1 thread, {c['patches']} randomly chosen patches "
                               f"of the workload ({c['sample_pairs']:.3e} pairs, "
                               f"{c['seconds']:.1f}s), scaled per pair to the full sweep"),
                    "pairs_per_s": c["pairs_per_s"]}
             log(f"cpu reference: {c['pairs_per_s']:.3e} pairs/s on {c['patches']} patches")
-        workers = min(16, len(os.sched_getaffinity(0)))
+        workers = min(CPU_SHARE, len(os.sched_getaffinity(0)))
         c = cpu_baseline_all_cores(patches, specs, thetas, args.cpu_seconds / 4, workers)
         if c is not None:
             t_sweep = pairs / c["pairs_per_s"]
             cpu_all = {"value": len(patches) / t_sweep, "unit": "evals/s", "cores": workers,
-                       "kind": "reference",
+                       "kind": "reference", "host": host_cpu(),
+                       "cores_note": (f"{workers} worker processes: the host's CPU share "
+                                      f"for one GPU is {CPU_SHARE} cores, whatever nproc "
+                                      f"reports"),
                        "sample": (f"forcepho's compute_gaussians_kernel.cc compiled from "
                                   f"source (g++ -O3), {workers} processes, {c['patches']} "
                                   f"randomly chosen patches ({c['sample_pairs']:.3e} pairs) "
@@ -483,7 +534,8 @@ def main():
         "plain_evals_per_s": plain,
         "batched_over_plain": (evals_per_s / plain) if plain else None,
         "chi2_rel_err_sample": chi2_err,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "hmc_graph": hmc_rate,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "hbm_GBps": (traffic / (eval_ms * 1e-3) / 1e9) if traffic else None,
@@ -491,9 +543,13 @@ def main():
                      "kernel": "eval pixel pass (eval_kernel / eval_persistent_kernel)",
                      "algorithmic_flop_per_sweep": flops_alg,
                      "pairs_in_cutoff": n_in,
+                     "unpacked_fp32_peak": PEAK_FP32_TFLOPS / 2,
+                     "frac_of_unpacked": achieved / (PEAK_FP32_TFLOPS / 2),
                      "note": ("compute-bound on the FP32 VALU (no dense contraction, so no "
-                              "MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s (the build "
-                              "uses no packed FP32); algorithmic FLOP = 98 per pair inside "
+                              "MFMA); peak = MI355X FP32 vector 157.3 TFLOP/s, which counts "
+                              "packed FP32; the build disables packed FP32, so one FMA per "
+                              "lane per 4 cycles issues at most half of it "
+                              "(unpacked_fp32_peak); algorithmic FLOP = 98 per pair inside "
                               "the cutoff + 11 per other pair + 7 N_pix n_src + 6 N_pix")},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
@@ -513,7 +569,6 @@ def run_config5(args):
     from forcepho_amd import _lib, synth
     from forcepho_amd.autotune import load_or_default
     from forcepho_amd.batch import DeviceSplines, PatchBatch
-    from forcepho_amd.hmc import HMC, DevicePosterior
     from forcepho_amd.limits import source_struct_dtype
 
     _lib.init(0)
@@ -585,33 +640,16 @@ def run_config5(args):
     batch.lnlike_and_grad(theta, splines)
     stage = np.array(batch.last_timing())
     batch.set_timing(False)
-    n_in = batch.count_pairs_in_cutoff()
+    import measure
+    n_in = measure.count_pairs_in_cutoff(batch)
     flops_alg = flops - FLOP_PER_PAIR * pairs + FLOP_PER_PAIR * n_in + \
         FLOP_PER_CUT_PAIR * max(pairs - n_in, 0)
     achieved = flops_alg / (stage[1] * 1e-3) / 1e12
 
     # HMC over every patch, one trajectory captured in a graph
-    lo, hi = [], []
-    for (p, sc, th) in patches:
-        l, h = synth.theta_bounds(th, len(p.band_N))
-        lo.append(l)
-        hi.append(h)
-    post = DevicePosterior(batch, splines, np.concatenate(lo), np.concatenate(hi))
-    z0 = post.inverse_transform(np.clip(np.concatenate(thetas), np.concatenate(lo) + 1e-9,
-                                        np.concatenate(hi) - 1e-9))
-    n_steps = 10
-    hmc = HMC(post, z0, step_size=1e-3, n_steps=n_steps, seed=5)
-    hmc.capture(warmup=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    n_traj = 2
-    hmc.run(n_traj)
-    torch.cuda.synchronize()
-    t_hmc = time.perf_counter() - t0
-    hmc_evals = len(patches) * n_steps * n_traj / t_hmc
-    accept = float(hmc.n_accept.mean().item()) / (n_traj + 1)   # + 1 warm-up
-    log(f"HMC: {n_traj} trajectories x {n_steps} steps in {t_hmc:.2f}s "
-        f"({hmc_evals:.0f} evals/s), mean acceptance {accept:.2f}")
+    hmc = time_hmc(patches, batch, splines, thetas, n_traj=2)
+    log(f"HMC: {hmc['trajectories']} trajectories x {hmc['leapfrog_steps']} steps, "
+        f"{hmc['evals_per_s']:.0f} evals/s, mean acceptance {hmc['acceptance']:.2f}")
 
     # plain drop-in on a random sample, scaled per pair to the sweep
     rng = np.random.default_rng(55)
@@ -635,11 +673,11 @@ def run_config5(args):
                    "launch": launch, "device_resident_gb": resident},
         "modes": modes,
         "pairs_in_cutoff": n_in,
-        "roofline_mixed": {"achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline_mixed": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                           "unit": "TFLOP/s",
                            "frac": achieved / PEAK_FP32_TFLOPS,
                            "pixel_pass_ms": float(stage[1])},
-        "hmc": {"trajectories": n_traj, "leapfrog_steps": n_steps, "seconds": t_hmc,
-                "evals_per_s": hmc_evals, "mean_acceptance": accept},
+        "hmc": hmc,
         "plain_evals_per_s": plain,
         "batched_over_plain": modes["mixed"]["evals_per_s"] / plain,
         "oracle_sample": {"patches": sample, "crowded_patch": crowded[:1]},

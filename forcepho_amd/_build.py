@@ -2,7 +2,9 @@
 
 The library is written next to this file so that it travels with the source
 tree; nothing is installed.  Packed FP32 instructions are disabled for all
-device code (``-packed-fp32-ops``).
+device code (``-packed-fp32-ops``).  ``build_measure_library`` builds the
+measurement-only ``measure/libfpho_measure.so`` that bench.py uses (issue
+calibration, in-cutoff pair count); the product never loads it.
 """
 
 import os
@@ -13,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBNAME = "libfpho_hip.so"
 LIBPATH = os.path.join(HERE, LIBNAME)
-SOURCES = ["kernels.hip", "runtime.hip", "calibrate.hip"]
+SOURCES = ["kernels.hip", "runtime.hip"]
 ARCH = "gfx950"
 
 
@@ -30,11 +32,13 @@ def compile_flags():
             "-fPIC", "-Wall", "-Wno-unused-function"]
 
 
-def build_library(verbose=False):
-    """Compile every HIP source into ``libfpho_hip.so`` (always recompiles)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIBPATH + ".tmp"
-    cmd = [hipcc_path()] + compile_flags() + ["-shared"] + srcs + ["-o", tmp]
+MEASURE_DIR = os.path.join(os.path.dirname(HERE), "measure")
+MEASURE_LIBPATH = os.path.join(MEASURE_DIR, "libfpho_measure.so")
+
+
+def _hipcc(cmd, out, verbose):
+    tmp = out + ".tmp"
+    cmd = cmd + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -45,9 +49,25 @@ def build_library(verbose=False):
         raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
     if verbose and msgs:
         print("\n".join(msgs))
-    os.replace(tmp, LIBPATH)
-    return LIBPATH
+    os.replace(tmp, out)
+    return out
+
+
+def build_library(verbose=False):
+    """Compile every HIP source into ``libfpho_hip.so`` (always recompiles)."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    return _hipcc([hipcc_path()] + compile_flags() +
+                  ["-shared", "-Wl,-soname,libfpho_hip.so"] + srcs, LIBPATH, verbose)
+
+
+def build_measure_library(verbose=False):
+    """Compile the measurement-only library against ``libfpho_hip.so``."""
+    return _hipcc([hipcc_path()] + compile_flags() +
+                  ["-shared", os.path.join(MEASURE_DIR, "measure.hip"), "-L" + HERE,
+                   "-lfpho_hip", "-Wl,-rpath,$ORIGIN/../forcepho_amd"],
+                  MEASURE_LIBPATH, verbose)
 
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_measure_library(verbose=True))

@@ -25,19 +25,30 @@ EXPORTED = [
     "fpho_last_error", "fpho_version", "fpho_init", "fpho_device_cu_count",
     "fpho_patch_create", "fpho_patch_set_meta", "fpho_patch_swap_data_residual",
     "fpho_patch_read", "fpho_patch_info", "fpho_patch_set_launch",
-    "fpho_patch_set_precision", "fpho_patch_free",
-    "fpho_evaluate", "fpho_evaluate_async",
+    "fpho_patch_set_precision", "fpho_patch_set_residual_writes", "fpho_patch_status",
+    "fpho_patch_free",
+    "fpho_evaluate", "fpho_evaluate_async", "fpho_evaluate_response",
+    "fpho_evaluate_response_async",
     "fpho_batch_create", "fpho_batch_layout", "fpho_batch_info",
     "fpho_batch_set_launch", "fpho_batch_set_precision",
     "fpho_batch_evaluate_async", "fpho_batch_set_timing", "fpho_batch_last_timing",
-    "fpho_batch_count_pairs", "fpho_issue_calibration",
-    "fpho_batch_free",
+    "fpho_batch_status", "fpho_batch_free",
     "fpho_spline_create", "fpho_spline_free",
     "fpho_batch_theta_to_proposals_async", "fpho_batch_stack_grad_async",
 ]
 
 ARRAY_IDS = {"data": 0, "ierr": 1, "xpix": 2, "ypix": 3, "residual": 4}
 PRECISION = {"fp32": 0, "fp64_reduce": 1, "mixed": 2}
+# FPHO_STATUS_* bits of the device status word
+STATUS_BITS = {0x1: "non-finite proposal value", 0x2: "axis ratio q <= 0",
+               0x4: "image Gaussian covariance not positive definite",
+               0x8: "non-finite image Gaussian parameter",
+               0x10: "non-finite parameter vector entry"}
+
+
+def describe_status(word):
+    """Names of the FPHO_STATUS_* bits set in ``word``."""
+    return [name for bit, name in STATUS_BITS.items() if word & bit]
 
 
 class FphoError(RuntimeError):
@@ -87,9 +98,13 @@ def _declare(lib):
         "fpho_patch_info": ([P, c_i32_p, c_i32_p, c_i64_p, c_i64_p], c_int),
         "fpho_patch_set_launch": ([P, ctypes.POINTER(Launch)], c_int),
         "fpho_patch_set_precision": ([P, c_int], c_int),
+        "fpho_patch_set_residual_writes": ([P, c_int], c_int),
+        "fpho_patch_status": ([P, ctypes.POINTER(ctypes.c_uint32), c_int], c_int),
         "fpho_patch_free": ([P], c_int),
         "fpho_evaluate": ([P, P, c_double_p, c_float_p], c_int),
         "fpho_evaluate_async": ([P, P, P, P, P], c_int),
+        "fpho_evaluate_response": ([P, P, c_double_p, P], c_int),
+        "fpho_evaluate_response_async": ([P, P, P, P, P], c_int),
         "fpho_batch_create": ([ctypes.POINTER(P), c_int, ctypes.POINTER(P)], c_int),
         "fpho_batch_layout": ([P, c_i64_p, c_i64_p, c_i64_p, c_i64_p], c_int),
         "fpho_batch_info": ([P, c_i64_p, c_i64_p, c_i64_p, c_i64_p], c_int),
@@ -98,9 +113,7 @@ def _declare(lib):
         "fpho_batch_evaluate_async": ([P, P, P, P, P], c_int),
         "fpho_batch_set_timing": ([P, c_int], c_int),
         "fpho_batch_last_timing": ([P, c_float_p], c_int),
-        "fpho_batch_count_pairs": ([P, P, c_i64_p], c_int),
-        "fpho_issue_calibration": ([c_int, c_double_p, c_double_p,
-                                    ctypes.POINTER(c_int)], c_int),
+        "fpho_batch_status": ([P, ctypes.POINTER(ctypes.c_uint32), c_int], c_int),
         "fpho_batch_free": ([P], c_int),
         "fpho_spline_create": ([c_double_p, c_int, c_double_p, c_int, c_double_p, c_int,
                                 c_double_p, ctypes.c_double, ctypes.POINTER(P)], c_int),

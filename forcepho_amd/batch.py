@@ -121,14 +121,13 @@ class PatchBatch:
         _lib.check(_lib.lib().fpho_batch_last_timing(self.handle, _lib.fptr(ms, ctypes.c_float)))
         return tuple(float(x) for x in ms)
 
-    def count_pairs_in_cutoff(self, proposals=None):
-        """(pixel, image Gaussian) pairs with exp argument <= 36 for the
-        proposals (default: the batch's buffer); synchronous."""
-        n = ctypes.c_int64()
-        props = self.proposals if proposals is None else proposals
-        _lib.check(_lib.lib().fpho_batch_count_pairs(self.handle, _lib.dptr(props),
-                                                     ctypes.byref(n)))
-        return n.value
+    def status(self, clear=True):
+        """The FPHO_STATUS_* bits the batch's kernels recorded since the last
+        cleared read (waits for the device); ``_lib.describe_status`` names
+        them."""
+        w = ctypes.c_uint32()
+        _lib.check(_lib.lib().fpho_batch_status(self.handle, ctypes.byref(w), int(bool(clear))))
+        return w.value
 
     def info(self):
         v = [ctypes.c_int64() for _ in range(4)]

@@ -44,17 +44,29 @@ struct SplineView {
 };
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes,
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
                           hipStream_t stream);
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
-hipError_t launch_count_pairs(const EvalArgs &a, unsigned long long *count, hipStream_t stream);
+hipError_t launch_response(const double *dchi2, int32_t n_bands, int32_t n_src,
+                           fpho_response *out, hipStream_t stream);
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,
                          const double *cpart, double *chi2, double *dchi2, int fp64,
                          hipStream_t stream);
 hipError_t launch_theta(const SrcDesc *srcs, int64_t n_srcs, const double *theta,
-                        const SplineView &sp, fpho_source *props, hipStream_t stream);
+                        const SplineView &sp, fpho_source *props, uint32_t *status,
+                        hipStream_t stream);
 hipError_t launch_stack(const SrcDesc *srcs, int64_t n_srcs, const PatchOut *pouts,
                         int64_t n_patches, const double *chi2, const double *dchi2,
                         double *lnlike, double *grad, hipStream_t stream);
+
+// A batch's current plan as the pixel pass sees it, for the measurement
+// library (libfpho_measure.so); not part of the C ABI.  Rebuilds a stale
+// plan first.
+struct PlanView {
+    EvalArgs eval;
+    int64_t n_exp;
+    uint32_t *status;
+};
+int batch_plan_view(fpho_batch *b, PlanView *out);
 
 }  // namespace fpho

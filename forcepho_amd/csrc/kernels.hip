@@ -30,10 +30,11 @@ namespace fpho {
 // GetGaussianAndJacobian (common_kernel.cc:175-240).  The per-Gaussian cost
 // is negligible next to the pixel pass, so it runs in double and is rounded
 // once into the FP32 record the pixel pass streams.
-__device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
-                                                    const fpho_source &src, int s,
-                                                    int p, ImgGauss &g,
-                                                    ImgGauss64 &g64) {
+// Returns the FPHO_STATUS_* bits of what it met (0 for a valid Gaussian).
+__device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e,
+                                                        const fpho_source &src, int s,
+                                                        int p, ImgGauss &g,
+                                                        ImgGauss64 &g64) {
     using M = M22<double>;
     const float *Dp = e.D + 4 * s;
     const float *CWp = e.CW + 4 * s;
@@ -139,12 +140,27 @@ __device__ __forceinline__ void make_image_gaussian(const ExpDesc &e,
     g64.hcs = sc2 * (24.0 - F.a - F.d);
     g64.ampA = amp / (24.0 * sc2);
     g64.pad0 = 0.0;
+
+    // what the reference would turn into NaN or Inf silently
+    uint32_t flags = 0;
+    if (!(isfinite(src.ra) && isfinite(src.dec) && isfinite(src.q) && isfinite(src.pa) &&
+          isfinite(flux) && isfinite(am) && isfinite(src.damplitude_dnsersic[bin]) &&
+          isfinite(src.damplitude_drh[bin])))
+        flags |= FPHO_STATUS_PROPOSAL_NONFINITE;
+    if (!(src.q > 0.0f)) flags |= FPHO_STATUS_Q_NONPOSITIVE;
+    if (!(det(cov) > 0.0)) flags |= FPHO_STATUS_COVARIANCE;
+    if (!(isfinite(amp) && isfinite(xc) && isfinite(yc) && isfinite(F.a) && isfinite(F.c) &&
+          isfinite(F.d) && isfinite(dA_dq) && isfinite(dA_dpa)))
+        flags |= FPHO_STATUS_GAUSSIAN_NONFINITE;
+    return flags;
 }
 
 // One workgroup per exposure; lanes stride over its n_src * n_psf Gaussians.
+// Invalid inputs set bits of *status (a vector atomic, taken only then).
 __global__ __launch_bounds__(128) void convert_kernel(
     const ExpDesc *__restrict__ exps, const fpho_source *__restrict__ props,
-    ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64, int4 *__restrict__ boxes) {
+    ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64, int4 *__restrict__ boxes,
+    uint32_t *__restrict__ status) {
     const ExpDesc e = exps[blockIdx.x];
     const int n = e.n_src * e.n_psf;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -152,7 +168,8 @@ __global__ __launch_bounds__(128) void convert_kernel(
         const int p = j - s * e.n_psf;
         ImgGauss g;
         ImgGauss64 g64;
-        make_image_gaussian(e, props[e.src_off + s], s, p, g, g64);
+        const uint32_t flags = make_image_gaussian(e, props[e.src_off + s], s, p, g, g64);
+        if (flags) atomicOr(status, flags);
         out[e.gauss_off + j] = g;
         out64[e.gauss_off + j] = g64;
         boxes[e.gauss_off + j] = make_int4(g.xlo, g.xhi, g.ylo, g.yhi);
@@ -591,46 +608,6 @@ __global__ __launch_bounds__(256) FPHO_EVAL_OCCUPANCY void eval_persistent_kerne
     }
 }
 
-// Number of (pixel, image Gaussian) pairs inside the cutoff (exp argument
-// <= 36) over the tiles, with the pixel pass's own FP32 test.  Measurement
-// only: it gives the reference algorithm's FLOP count for the roofline.
-template <int K>
-__global__ __launch_bounds__(256) void count_pairs_kernel(
-    const TileDesc *__restrict__ tiles, int32_t n_tiles, int32_t waves_per_group,
-    const ExpDesc *__restrict__ exps, const ImgGauss *__restrict__ gauss,
-    unsigned long long *__restrict__ count) {
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = blockIdx.x * waves_per_group + w;
-    if (tile >= n_tiles) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    const TileDesc t = tiles[tile];
-    const ExpDesc &e = exps[t.exp];
-    const ImgGauss *__restrict__ g = gauss + e.gauss_off;
-    const int ng = e.n_src * e.n_psf;
-    float x[K], y[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int p = k * kWave + lane;
-        x[k] = p < t.npix ? e.xpix[t.pix0 + p] - e.x0 : kMaskedCoord;
-        y[k] = p < t.npix ? e.ypix[t.pix0 + p] - e.y0 : kMaskedCoord;
-    }
-    const int4 tbox = make_int4(t.xlo, t.xhi, t.ylo, t.yhi);
-    unsigned long long n = 0;
-    for (int i = 0; i < ng; ++i) {
-        if (boxed_out(1, *reinterpret_cast<const int4 *>(&g[i].xlo), tbox)) continue;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const float dx = x[k] - g[i].xcen;
-            const float dy = y[k] - g[i].ycen;
-            const float vx = g[i].sfxx * dx + g[i].sfxy * dy;
-            const float vy = g[i].sfyy * dy + g[i].sfxy * dx;
-            const float ea = dx * vx + dy * vy;
-            n += __popcll(__builtin_amdgcn_ballot_w64(!(ea < kCutScaled)));
-        }
-    }
-    if (lane == 0) atomicAdd(count, n);
-}
-
 __global__ void zero_counter_kernel(int32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
@@ -741,11 +718,16 @@ __device__ inline int knot_span(const double *t, int n_t, double x) {
 
 __global__ __launch_bounds__(64) void theta_kernel(
     const SrcDesc *__restrict__ srcs, int32_t n_srcs, const double *__restrict__ theta,
-    SplineView sp, fpho_source *__restrict__ props) {
+    SplineView sp, fpho_source *__restrict__ props, uint32_t *__restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_srcs) return;
     const SrcDesc sd = srcs[i];
     const double *th = theta + sd.theta_off;
+    {
+        bool finite = true;
+        for (int j = 0; j < sd.n_bands + 6; ++j) finite = finite && isfinite(th[j]);
+        if (!finite) atomicOr(status, FPHO_STATUS_THETA_NONFINITE);
+    }
     fpho_source o;
     for (int b = 0; b < FPHO_MAXBANDS; ++b) o.fluxes[b] = 0.0f;
     for (int m = 0; m < FPHO_MAXRADII; ++m) {
@@ -837,13 +819,36 @@ __global__ __launch_bounds__(64) void stack_kernel(
     }
 }
 
+// ------------------------------------------------------ compat Response out
+
+// dchi2 [n_bands][n_src][7] (double) -> Response[n_bands] (float, stride
+// MAXSOURCES * 7, compute_gaussians_kernel.cu:304-310), tail zeroed.
+__global__ __launch_bounds__(256) void response_kernel(const double *__restrict__ dchi2,
+                                                       int32_t n_src,
+                                                       fpho_response *__restrict__ out) {
+    const int b = blockIdx.x;
+    constexpr int kStride = FPHO_MAXSOURCES * kNParams;
+    for (int j = threadIdx.x; j < kStride; j += blockDim.x)
+        out[b].dchi2_dparam[j] =
+            j < n_src * kNParams ? (float)dchi2[(int64_t)b * n_src * kNParams + j] : 0.0f;
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
-                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, hipStream_t stream) {
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
+                          hipStream_t stream) {
     if (n_exp <= 0) return hipSuccess;
     hipLaunchKernelGGL(convert_kernel, dim3((unsigned)n_exp), dim3(128), 0, stream,
-                       exps, props, out, out64, boxes);
+                       exps, props, out, out64, boxes, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_response(const double *dchi2, int32_t n_bands, int32_t n_src,
+                           fpho_response *out, hipStream_t stream) {
+    if (n_bands <= 0) return hipSuccess;
+    hipLaunchKernelGGL(response_kernel, dim3((unsigned)n_bands), dim3(256), 0, stream, dchi2,
+                       n_src, out);
     return hipGetLastError();
 }
 
@@ -885,34 +890,6 @@ hipError_t launch_eval(const EvalArgs &a, hipStream_t stream) {
                                      : launch_eval_p<false>(a, stream);
 }
 
-hipError_t launch_count_pairs(const EvalArgs &a, unsigned long long *count,
-                              hipStream_t stream) {
-    if (a.n_tiles <= 0) return hipSuccess;
-    const int W = a.waves_per_group;
-    const unsigned groups = (unsigned)((a.n_tiles + W - 1) / W);
-    switch (a.pix_per_lane) {
-        case 1:
-            hipLaunchKernelGGL(count_pairs_kernel<1>, dim3(groups), dim3(64 * W), 0, stream,
-                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
-            break;
-        case 2:
-            hipLaunchKernelGGL(count_pairs_kernel<2>, dim3(groups), dim3(64 * W), 0, stream,
-                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
-            break;
-        case 4:
-            hipLaunchKernelGGL(count_pairs_kernel<4>, dim3(groups), dim3(64 * W), 0, stream,
-                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
-            break;
-        case 8:
-            hipLaunchKernelGGL(count_pairs_kernel<8>, dim3(groups), dim3(64 * W), 0, stream,
-                               a.tiles, (int32_t)a.n_tiles, (int32_t)W, a.exps, a.gauss, count);
-            break;
-        default:
-            return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,
                          const double *cpart, double *chi2, double *dchi2, int fp64,
                          hipStream_t stream) {
@@ -927,11 +904,12 @@ hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart
 }
 
 hipError_t launch_theta(const SrcDesc *srcs, int64_t n_srcs, const double *theta,
-                        const SplineView &sp, fpho_source *props, hipStream_t stream) {
+                        const SplineView &sp, fpho_source *props, uint32_t *status,
+                        hipStream_t stream) {
     if (n_srcs <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n_srcs + 63) / 64);
     hipLaunchKernelGGL(theta_kernel, dim3(blocks), dim3(64), 0, stream, srcs, (int32_t)n_srcs,
-                       theta, sp, props);
+                       theta, sp, props, status);
     return hipGetLastError();
 }
 

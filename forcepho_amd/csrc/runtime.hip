@@ -20,7 +20,6 @@
 
 #include "../../include/fpho.h"
 #include "fpho_internal.h"
-#include "calibrate.h"
 #include "kernels.h"
 
 using namespace fpho;
@@ -163,6 +162,8 @@ struct Plan {
     DevVec<float> gpart;
     DevVec<double> cpart;
     DevVec<int32_t> counter;
+    DevVec<uint32_t> status;        // FPHO_STATUS_* bits of the plan's kernels
+    DevVec<double> dchi2_tmp;       // single patch: dchi2 before the compat Response layout
     DevVec<SrcDesc> srcs;
     DevVec<PatchOut> pouts;
     int64_t n_exp = 0, n_tiles = 0, n_outs = 0, n_gauss = 0, n_srcs = 0, n_patches = 0;
@@ -197,8 +198,9 @@ struct fpho_patch {
     DevVec<float> data, ierr, xpix, ypix, residual;
     DevVec<float> rad2, D, crpix, crval, CW;
     DevVec<fpho_psf_gauss> psf;
-    bool return_residual = false;
-    uint64_t generation = 1;  // bumps on metadata change or data/residual swap
+    bool write_residual = false;  // evaluations write residual = data - model
+    uint64_t generation = 1;  // bumps on metadata change, a data/residual swap or
+                              // a residual-write change
     fpho_launch launch = default_launch(false);
     int precision = kPrecMixed;
     Plan plan;
@@ -215,6 +217,9 @@ struct fpho_patch {
 struct fpho_batch {
     std::vector<fpho_patch *> patches;
     std::vector<uint64_t> generations;
+    std::vector<int32_t> n_sources0;  // each patch's source count at creation
+    fpho_launch launch = default_launch(true);
+    int precision = kPrecMixed;
     Plan plan;
 };
 
@@ -352,7 +357,7 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
             ed.ierr = p->ierr.ptr;
             ed.xpix = p->xpix.ptr;
             ed.ypix = p->ypix.ptr;
-            ed.residual = p->return_residual ? p->residual.ptr : nullptr;
+            ed.residual = p->write_residual ? p->residual.ptr : nullptr;
             ed.rad2 = p->rad2.ptr;
             ed.D = p->D.ptr + (int64_t)4 * ns * e;
             ed.CW = p->CW.ptr + (int64_t)4 * ns * e;
@@ -428,8 +433,10 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
     hp.pouts.push_back(po);
 }
 
+// On failure the plan stays invalid (its buffers may be partly replaced).
 int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &launch,
                Plan &plan) {
+    plan.valid = false;
     HostPlan hp;
     plan.src_off.assign(1, 0);
     plan.chi2_off.assign(1, 0);
@@ -469,6 +476,14 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     HIP_TRY(plan.cpart.alloc(std::max<int64_t>((int64_t)hp.tiles.size(), 1)));
     HIP_TRY(plan.counter.alloc(1));
     HIP_TRY(hipMemset(plan.counter.ptr, 0, sizeof(int32_t)));
+    HIP_TRY(plan.status.alloc(1));
+    HIP_TRY(hipMemset(plan.status.ptr, 0, sizeof(uint32_t)));
+    if (patches.size() == 1) {
+        const fpho_patch *p = patches[0];
+        HIP_TRY(plan.dchi2_tmp.alloc((int64_t)p->n_bands * p->n_sources * kNParams));
+    } else {
+        plan.dchi2_tmp.release();
+    }
     plan.n_exp = (int64_t)hp.exps.size();
     plan.n_tiles = (int64_t)hp.tiles.size();
     plan.n_outs = (int64_t)hp.outs.size();
@@ -514,7 +529,7 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
              hipStream_t stream) {
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
     HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
-                           plan.gboxes.ptr, stream));
+                           plan.gboxes.ptr, plan.status.ptr, stream));
     const EvalArgs a = eval_args(plan);
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[1], stream));
     HIP_TRY(launch_eval(a, stream));
@@ -525,8 +540,20 @@ int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
     return FPHO_OK;
 }
 
-int ensure_patch_plan(fpho_patch *p) {
+// True when `stream` is capturing a graph (plans are not rebuilt then: a
+// rebuild allocates and copies synchronously).
+bool capturing(hipStream_t stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) return false;
+    return st != hipStreamCaptureStatusNone;
+}
+
+int ensure_patch_plan(fpho_patch *p, hipStream_t stream) {
     if (p->plan.valid && p->plan_generation == p->generation) return FPHO_OK;
+    if (capturing(stream))
+        return fail(FPHO_E_STATE,
+                    "the patch changed since its last evaluation and its plan must be "
+                    "rebuilt (allocates); evaluate it once outside the graph capture first");
     std::vector<fpho_patch *> one{p};
     const int rc = build_plan(one, p->launch, p->plan);
     if (rc) return rc;
@@ -535,17 +562,78 @@ int ensure_patch_plan(fpho_patch *p) {
     return FPHO_OK;
 }
 
-int ensure_batch_plan(fpho_batch *b) {
+int ensure_batch_plan(fpho_batch *b, hipStream_t stream) {
+    for (size_t i = 0; i < b->patches.size(); ++i)
+        if (b->patches[i]->n_sources != b->n_sources0[i])
+            return fail(FPHO_E_STATE,
+                        "patch %zu of the batch now has %d sources (%d when the batch was "
+                        "created); the batch layout no longer holds: create a new batch",
+                        i, b->patches[i]->n_sources, b->n_sources0[i]);
     bool stale = !b->plan.valid;
     for (size_t i = 0; i < b->patches.size() && !stale; ++i)
         stale = b->generations[i] != b->patches[i]->generation;
     if (!stale) return FPHO_OK;
-    const fpho_launch l = b->plan.valid ? b->plan.launch : default_launch(true);
-    const int prec = b->plan.precision;
-    const int rc = build_plan(b->patches, l, b->plan);
+    if (capturing(stream))
+        return fail(FPHO_E_STATE,
+                    "a patch of the batch changed since the last evaluation and the plan must "
+                    "be rebuilt (allocates); evaluate once outside the graph capture first");
+    const int rc = build_plan(b->patches, b->launch, b->plan);
     if (rc) return rc;
-    b->plan.precision = prec;
+    b->plan.precision = b->precision;
     for (size_t i = 0; i < b->patches.size(); ++i) b->generations[i] = b->patches[i]->generation;
+    return FPHO_OK;
+}
+
+int read_status(Plan &plan, uint32_t *status, int clear) {
+    *status = 0;
+    if (!plan.valid || !plan.status.ptr) return FPHO_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(status, plan.status.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (clear) HIP_TRY(hipMemset(plan.status.ptr, 0, sizeof(uint32_t)));
+    return FPHO_OK;
+}
+
+int check_proposal_host(const fpho_patch *p, const fpho_source *prop) {
+    for (int s = 0; s < p->n_sources; ++s) {
+        const fpho_source &src = prop[s];
+        if (!(src.q > 0.0f) || !std::isfinite(src.q) || !std::isfinite(src.pa) ||
+            !std::isfinite(src.ra) || !std::isfinite(src.dec))
+            return fail(FPHO_E_VALUE, "source %d: q must be > 0 and positions finite", s);
+        for (int b = 0; b < p->n_bands; ++b)
+            if (!std::isfinite(src.fluxes[b]))
+                return fail(FPHO_E_VALUE, "source %d: non-finite flux in band %d", s, b);
+        for (int m = 0; m < p->n_radii; ++m)
+            if (!std::isfinite(src.mixture_amplitudes[m]) ||
+                !std::isfinite(src.damplitude_drh[m]) ||
+                !std::isfinite(src.damplitude_dnsersic[m]))
+                return fail(FPHO_E_VALUE, "source %d: non-finite mixture amplitude", s);
+    }
+    return FPHO_OK;
+}
+
+// Synchronous evaluation through the patch's page-locked staging: host
+// proposal in, chi2 [n_bands] and dchi2 [n_bands][n_sources][7] (double) out
+// in out_host.
+int evaluate_sync(fpho_patch *p, const fpho_source *prop) {
+    int rc = check_proposal_host(p, prop);
+    if (rc) return rc;
+    rc = ensure_patch_plan(p, nullptr);
+    if (rc) return rc;
+    const int64_t nd = (int64_t)p->n_bands * p->n_sources * kNParams;
+    const int64_t nout = p->n_bands + nd;
+    if (p->prop_dev.n < p->n_sources) HIP_TRY(p->prop_dev.alloc(p->n_sources));
+    if (p->out_dev.n < nout) HIP_TRY(p->out_dev.alloc(nout));
+    if (p->prop_host.n < p->n_sources) HIP_TRY(p->prop_host.alloc(p->n_sources));
+    if (p->out_host.n < nout) HIP_TRY(p->out_host.alloc(nout));
+    std::memcpy(p->prop_host.ptr, prop, sizeof(fpho_source) * p->n_sources);
+    HIP_TRY(hipMemcpyAsync(p->prop_dev.ptr, p->prop_host.ptr,
+                           sizeof(fpho_source) * p->n_sources, hipMemcpyHostToDevice, nullptr));
+    rc = run_plan(p->plan, p->prop_dev.ptr, p->out_dev.ptr, p->out_dev.ptr + p->n_bands,
+                  nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(p->out_host.ptr, p->out_dev.ptr, sizeof(double) * nout,
+                           hipMemcpyDeviceToHost, nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
     return FPHO_OK;
 }
 
@@ -592,7 +680,7 @@ int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch 
     p->band_N.assign(d->band_N, d->band_N + d->n_bands);
     p->exposure_start.assign(d->exposure_start, d->exposure_start + d->n_exp);
     p->exposure_N.assign(d->exposure_N, d->exposure_N + d->n_exp);
-    p->return_residual = return_residual != 0;
+    p->write_residual = return_residual != 0;
     p->chunk_start.assign(1, 0);
     for (int ex = 0; ex < d->n_exp; ++ex) {
         const int64_t s0 = d->exposure_start[ex], n = d->exposure_N[ex];
@@ -624,7 +712,7 @@ int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch 
         delete p;
         return fail(FPHO_E_HIP, "pixel upload failed: %s", hipGetErrorString(e));
     }
-    if (p->return_residual) {
+    if (p->write_residual) {
         if ((e = p->residual.alloc(d->npix)) != hipSuccess ||
             (e = hipMemset(p->residual.ptr, 0, sizeof(float) * d->npix)) != hipSuccess) {
             delete p;
@@ -654,8 +742,9 @@ int fpho_patch_set_meta(fpho_patch *p, const fpho_patch_desc *d) {
 
 int fpho_patch_swap_data_residual(fpho_patch *p) {
     if (!p) return fail(FPHO_E_ARG, "null patch");
-    if (!p->return_residual)
-        return fail(FPHO_E_STATE, "patch was created without a residual buffer");
+    if (!p->residual.ptr)
+        return fail(FPHO_E_STATE, "patch has no residual buffer (create it with "
+                                  "return_residual or call fpho_patch_set_residual_writes)");
     p->data.swap(p->residual);
     ++p->generation;
     return FPHO_OK;
@@ -711,6 +800,25 @@ int fpho_patch_set_precision(fpho_patch *p, int mode) {
     return FPHO_OK;
 }
 
+int fpho_patch_set_residual_writes(fpho_patch *p, int on) {
+    if (!p) return fail(FPHO_E_ARG, "null patch");
+    const bool w = on != 0;
+    if (w && !p->residual.ptr) {
+        HIP_TRY(p->residual.alloc(p->npix));
+        HIP_TRY(hipMemset(p->residual.ptr, 0, sizeof(float) * p->npix));
+    }
+    if (w != p->write_residual) {
+        p->write_residual = w;
+        ++p->generation;
+    }
+    return FPHO_OK;
+}
+
+int fpho_patch_status(fpho_patch *p, uint32_t *status, int clear) {
+    if (!p || !status) return fail(FPHO_E_ARG, "null argument");
+    return read_status(p->plan, status, clear);
+}
+
 int fpho_patch_free(fpho_patch *p) {
     if (!p) return FPHO_OK;
     (void)hipDeviceSynchronize();
@@ -720,37 +828,9 @@ int fpho_patch_free(fpho_patch *p) {
 
 int fpho_evaluate(fpho_patch *p, const fpho_source *prop, double *chi2, float *dchi2) {
     if (!p || !prop || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
-    for (int s = 0; s < p->n_sources; ++s) {
-        const fpho_source &src = prop[s];
-        if (!(src.q > 0.0f) || !std::isfinite(src.q) || !std::isfinite(src.pa) ||
-            !std::isfinite(src.ra) || !std::isfinite(src.dec))
-            return fail(FPHO_E_VALUE, "source %d: q must be > 0 and positions finite", s);
-        for (int b = 0; b < p->n_bands; ++b)
-            if (!std::isfinite(src.fluxes[b]))
-                return fail(FPHO_E_VALUE, "source %d: non-finite flux in band %d", s, b);
-        for (int m = 0; m < p->n_radii; ++m)
-            if (!std::isfinite(src.mixture_amplitudes[m]) ||
-                !std::isfinite(src.damplitude_drh[m]) ||
-                !std::isfinite(src.damplitude_dnsersic[m]))
-                return fail(FPHO_E_VALUE, "source %d: non-finite mixture amplitude", s);
-    }
-    int rc = ensure_patch_plan(p);
+    const int rc = evaluate_sync(p, prop);
     if (rc) return rc;
     const int64_t nd = (int64_t)p->n_bands * p->n_sources * kNParams;
-    const int64_t nout = p->n_bands + nd;
-    if (p->prop_dev.n < p->n_sources) HIP_TRY(p->prop_dev.alloc(p->n_sources));
-    if (p->out_dev.n < nout) HIP_TRY(p->out_dev.alloc(nout));
-    if (p->prop_host.n < p->n_sources) HIP_TRY(p->prop_host.alloc(p->n_sources));
-    if (p->out_host.n < nout) HIP_TRY(p->out_host.alloc(nout));
-    std::memcpy(p->prop_host.ptr, prop, sizeof(fpho_source) * p->n_sources);
-    HIP_TRY(hipMemcpyAsync(p->prop_dev.ptr, p->prop_host.ptr,
-                           sizeof(fpho_source) * p->n_sources, hipMemcpyHostToDevice, nullptr));
-    rc = run_plan(p->plan, p->prop_dev.ptr, p->out_dev.ptr, p->out_dev.ptr + p->n_bands,
-                  nullptr);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(p->out_host.ptr, p->out_dev.ptr, sizeof(double) * nout,
-                           hipMemcpyDeviceToHost, nullptr));
-    HIP_TRY(hipStreamSynchronize(nullptr));
     std::memcpy(chi2, p->out_host.ptr, sizeof(double) * p->n_bands);
     for (int64_t i = 0; i < nd; ++i) dchi2[i] = (float)p->out_host.ptr[p->n_bands + i];
     return FPHO_OK;
@@ -759,9 +839,42 @@ int fpho_evaluate(fpho_patch *p, const fpho_source *prop, double *chi2, float *d
 int fpho_evaluate_async(fpho_patch *p, const fpho_source *prop_dev, double *chi2_dev,
                         double *dchi2_dev, void *stream) {
     if (!p || !prop_dev || !chi2_dev || !dchi2_dev) return fail(FPHO_E_ARG, "null argument");
-    int rc = ensure_patch_plan(p);
+    int rc = ensure_patch_plan(p, (hipStream_t)stream);
     if (rc) return rc;
     return run_plan(p->plan, prop_dev, chi2_dev, dchi2_dev, (hipStream_t)stream);
+}
+
+int fpho_evaluate_response(fpho_patch *p, const fpho_source *prop, double *chi2,
+                           fpho_response *resp) {
+    if (!p || !prop || !chi2 || !resp) return fail(FPHO_E_ARG, "null argument");
+    if (p->n_sources > FPHO_MAXSOURCES)
+        return fail(FPHO_E_LIMIT, "%d sources do not fit the Response layout (MAXSOURCES %d)",
+                    p->n_sources, FPHO_MAXSOURCES);
+    const int rc = evaluate_sync(p, prop);
+    if (rc) return rc;
+    std::memcpy(chi2, p->out_host.ptr, sizeof(double) * p->n_bands);
+    const int nw = p->n_sources * kNParams;
+    for (int b = 0; b < p->n_bands; ++b) {
+        const double *src = p->out_host.ptr + p->n_bands + (int64_t)b * nw;
+        for (int j = 0; j < FPHO_MAXSOURCES * kNParams; ++j)
+            resp[b].dchi2_dparam[j] = j < nw ? (float)src[j] : 0.0f;
+    }
+    return FPHO_OK;
+}
+
+int fpho_evaluate_response_async(fpho_patch *p, const fpho_source *prop_dev, double *chi2_dev,
+                                 fpho_response *resp_dev, void *stream) {
+    if (!p || !prop_dev || !chi2_dev || !resp_dev) return fail(FPHO_E_ARG, "null argument");
+    if (p->n_sources > FPHO_MAXSOURCES)
+        return fail(FPHO_E_LIMIT, "%d sources do not fit the Response layout (MAXSOURCES %d)",
+                    p->n_sources, FPHO_MAXSOURCES);
+    int rc = ensure_patch_plan(p, (hipStream_t)stream);
+    if (rc) return rc;
+    rc = run_plan(p->plan, prop_dev, chi2_dev, p->plan.dchi2_tmp.ptr, (hipStream_t)stream);
+    if (rc) return rc;
+    HIP_TRY(launch_response(p->plan.dchi2_tmp.ptr, p->n_bands, p->n_sources, resp_dev,
+                            (hipStream_t)stream));
+    return FPHO_OK;
 }
 
 int fpho_batch_create(fpho_patch *const *patches, int n, fpho_batch **out) {
@@ -776,8 +889,9 @@ int fpho_batch_create(fpho_patch *const *patches, int n, fpho_batch **out) {
         }
         b->patches.push_back(patches[i]);
         b->generations.push_back(0);
+        b->n_sources0.push_back(patches[i]->n_sources);
     }
-    const int rc = ensure_batch_plan(b);
+    const int rc = ensure_batch_plan(b, nullptr);
     if (rc) {
         delete b;
         return rc;
@@ -813,15 +927,15 @@ int fpho_batch_set_launch(fpho_batch *b, const fpho_launch *l) {
     if (!b || !l) return fail(FPHO_E_ARG, "null argument");
     const int rc = check_launch(*l);
     if (rc) return rc;
-    const int prec = b->plan.precision;
-    const int r2 = build_plan(b->patches, *l, b->plan);
-    b->plan.precision = prec;
-    return r2;
+    b->launch = *l;
+    b->plan.valid = false;    // rebuilt now; on failure, again at the next call
+    return ensure_batch_plan(b, nullptr);
 }
 
 int fpho_batch_set_precision(fpho_batch *b, int mode) {
     if (!b) return fail(FPHO_E_ARG, "null batch");
     if (mode < kPrecFP32 || mode > kPrecMixed) return fail(FPHO_E_ARG, "precision mode %d", mode);
+    b->precision = mode;
     b->plan.precision = mode;
     return FPHO_OK;
 }
@@ -829,7 +943,7 @@ int fpho_batch_set_precision(fpho_batch *b, int mode) {
 int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *props, double *chi2,
                               double *dchi2, void *stream) {
     if (!b || !props || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
-    const int rc = ensure_batch_plan(b);
+    const int rc = ensure_batch_plan(b, (hipStream_t)stream);
     if (rc) return rc;
     return run_plan(b->plan, props, chi2, dchi2, (hipStream_t)stream);
 }
@@ -853,56 +967,9 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
     return FPHO_OK;
 }
 
-int fpho_issue_calibration(int kind, double *ms, double *wave_instructions, int *clock_khz) {
-    if (!ms || !wave_instructions || !clock_khz) return fail(FPHO_E_ARG, "null argument");
-    if (kind < 0 || kind > 6) return fail(FPHO_E_ARG, "kind must be 0 to 6 (got %d)", kind);
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    int cus = 0, clk = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev));
-    const int blocks = 8 * cus;   // 8 workgroups of 4 waves: 8 waves per SIMD
-    DevVec<float> out;
-    HIP_TRY(out.alloc((int64_t)blocks * 256));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    int64_t per_wave = 0;
-    float t = 0.0f;
-    hipError_t err = hipSuccess;
-    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {
-        if ((err = hipEventRecord(e0, nullptr)) != hipSuccess) break;
-        if ((err = launch_issue_calibration(kind, blocks, out.ptr, nullptr, &per_wave)) !=
-            hipSuccess)
-            break;
-        if ((err = hipEventRecord(e1, nullptr)) != hipSuccess) break;
-        if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
-        err = hipEventElapsedTime(&t, e0, e1);
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (err != hipSuccess)
-        return fail(FPHO_E_HIP, "issue calibration failed: %s", hipGetErrorString(err));
-    *ms = t;
-    *wave_instructions = (double)blocks * 4.0 * (double)per_wave;
-    *clock_khz = clk;
-    return FPHO_OK;
-}
-
-int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *props, int64_t *n_in) {
-    if (!b || !props || !n_in) return fail(FPHO_E_ARG, "null argument");
-    int rc = ensure_batch_plan(b);
-    if (rc) return rc;
-    DevVec<unsigned long long> cnt;
-    HIP_TRY(cnt.alloc(1));
-    HIP_TRY(hipMemset(cnt.ptr, 0, sizeof(unsigned long long)));
-    HIP_TRY(launch_convert(b->plan.exps.ptr, b->plan.n_exp, props, b->plan.gauss.ptr,
-                           b->plan.gauss64.ptr, b->plan.gboxes.ptr, nullptr));
-    HIP_TRY(launch_count_pairs(eval_args(b->plan), cnt.ptr, nullptr));
-    unsigned long long h = 0;
-    HIP_TRY(hipMemcpy(&h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost));
-    *n_in = (int64_t)h;
-    return FPHO_OK;
+int fpho_batch_status(fpho_batch *b, uint32_t *status, int clear) {
+    if (!b || !status) return fail(FPHO_E_ARG, "null argument");
+    return read_status(b->plan, status, clear);
 }
 
 int fpho_batch_free(fpho_batch *b) {
@@ -958,17 +1025,17 @@ int fpho_batch_theta_to_proposals_async(fpho_batch *b, const fpho_spline *s,
                                         const double *theta, fpho_source *props,
                                         void *stream) {
     if (!b || !s || !theta || !props) return fail(FPHO_E_ARG, "null argument");
-    const int rc = ensure_batch_plan(b);
+    const int rc = ensure_batch_plan(b, (hipStream_t)stream);
     if (rc) return rc;
     HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props,
-                         (hipStream_t)stream));
+                         b->plan.status.ptr, (hipStream_t)stream));
     return FPHO_OK;
 }
 
 int fpho_batch_stack_grad_async(fpho_batch *b, const double *chi2, const double *dchi2,
                                 double *lnlike, double *grad, void *stream) {
     if (!b || !chi2 || !dchi2 || !lnlike || !grad) return fail(FPHO_E_ARG, "null argument");
-    const int rc = ensure_batch_plan(b);
+    const int rc = ensure_batch_plan(b, (hipStream_t)stream);
     if (rc) return rc;
     HIP_TRY(launch_stack(b->plan.srcs.ptr, b->plan.n_srcs, b->plan.pouts.ptr,
                          b->plan.n_patches, chi2, dchi2, lnlike, grad, (hipStream_t)stream));
@@ -976,3 +1043,17 @@ int fpho_batch_stack_grad_async(fpho_batch *b, const double *chi2, const double 
 }
 
 }  // extern "C"
+
+// Internal hook of the measurement library (declared in kernels.h, not in the
+// C ABI).
+namespace fpho {
+int batch_plan_view(fpho_batch *b, PlanView *out) {
+    if (!b || !out) return fail(FPHO_E_ARG, "null argument");
+    const int rc = ensure_batch_plan(b, nullptr);
+    if (rc) return rc;
+    out->eval = eval_args(b->plan);
+    out->n_exp = b->plan.n_exp;
+    out->status = b->plan.status.ptr;
+    return FPHO_OK;
+}
+}  // namespace fpho

@@ -214,6 +214,24 @@ class HIPPatchMixin(DevicePatchMixin):
         _lib.check(_lib.lib().fpho_patch_set_precision(self.device_patch,
                                                        _lib.PRECISION[mode]))
 
+    def set_residual_writes(self, on):
+        """Start or stop writing the residual at every evaluation (and
+        returning it from ``evaluate_proposal``); the residual buffer is
+        kept, so ``swap_on_device`` still works after stopping."""
+        _lib.check(_lib.lib().fpho_patch_set_residual_writes(self.device_patch,
+                                                             int(bool(on))))
+        self.return_residual = bool(on)
+        if on:
+            self._has_residual = True
+
+    def device_status(self, clear=True):
+        """FPHO_STATUS_* bits recorded by this patch's stream-ordered
+        evaluations since the last cleared read (waits for the device)."""
+        w = ctypes.c_uint32()
+        _lib.check(_lib.lib().fpho_patch_status(self.device_patch, ctypes.byref(w),
+                                                int(bool(clear))))
+        return w.value
+
     def free(self):
         handle = getattr(self, "device_patch", None)
         if handle:

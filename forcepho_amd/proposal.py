@@ -65,6 +65,26 @@ class HIPProposer(ProposerBase):
             return chi_out, derivs, residuals
         return chi_out, derivs
 
+    def evaluate_response(self, proposal, patch=None):
+        """The reference kernel's raw output layout: ``chi2`` (float64
+        [n_bands]) and ``Response[n_bands]`` (``response_struct_dtype``,
+        forcepho/proposal.py:242-244), i.e. what ``EvaluateProposal`` writes
+        into ``cuda.Out(chi_derivs_out)`` before ``evaluate_proposal``
+        reshapes it to [n_bands, MAXSOURCES, NPARAMS] (forcepho/proposal.py:
+        119-141).  Entries past 7 n_sources of each band are zero."""
+        if patch is None:
+            patch = self.patch
+        n_bands, n_sources = int(len(patch.band_N)), int(patch.n_sources)
+        proposal = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+        if len(proposal) != n_sources:
+            raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
+        chi_out = np.empty(n_bands, dtype=np.float64)
+        resp = np.empty(n_bands, dtype=response_struct_dtype)
+        _lib.check(_lib.lib().fpho_evaluate_response(
+            patch.device_patch, ctypes.c_void_p(proposal.ctypes.data),
+            _lib.fptr(chi_out, ctypes.c_double), ctypes.c_void_p(resp.ctypes.data)))
+        return chi_out.astype(self.chi_dtype), resp
+
 
 # names used by the reference for its GPU proposer
 GPUProposer = HIPProposer

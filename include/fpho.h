@@ -86,6 +86,28 @@ typedef struct fpho_psf_gauss {
     int32_t sersic_radius_bin;
 } fpho_psf_gauss;
 
+/* == class Response, forcepho/src/proposal.cu:33-39 (840 bytes): the
+ * gradients of one band, dchi2_dparam[source * 7 + param] with params in the
+ * order d/d(flux, ra, dec, q, pa, nsersic, rh).  pycuda callers allocate
+ * n_bands of these (response_struct_dtype, forcepho/proposal.py:242-244) and
+ * reshape them to [n_bands, MAXSOURCES, 7] (forcepho/proposal.py:139-141). */
+typedef struct fpho_response {
+    float dchi2_dparam[FPHO_MAXSOURCES * FPHO_NPARAMS];
+} fpho_response;
+
+/* Bits of the device status word.  The reference kernel turns such inputs
+ * into NaN or Inf silently (forcepho/src/common_kernel.cc:175-240); the
+ * stream-ordered entry points cannot return an error for values that only
+ * exist on the device, so their kernels record what they met here and
+ * fpho_patch_status / fpho_batch_status read it after the work finished.
+ * The results of an evaluation that set a bit are undefined (usually NaN). */
+#define FPHO_STATUS_PROPOSAL_NONFINITE 0x1u  /* NaN/Inf in ra, dec, q, pa, flux or amplitudes */
+#define FPHO_STATUS_Q_NONPOSITIVE 0x2u       /* axis ratio q <= 0                          */
+#define FPHO_STATUS_COVARIANCE 0x4u          /* image Gaussian covariance not positive
+                                                definite (det <= 0 or NaN)                */
+#define FPHO_STATUS_GAUSSIAN_NONFINITE 0x8u  /* non-finite image Gaussian parameter        */
+#define FPHO_STATUS_THETA_NONFINITE 0x10u    /* non-finite parameter vector entry           */
+
 /* Host-side description of one patch: the arrays of forcepho/patches/patch.py:
  * 121-197 (PatchBase.build_patch contract) with the struct layout of
  * forcepho/src/patch.cc:23-100.  All pointers are HOST pointers, read only
@@ -155,6 +177,16 @@ int fpho_patch_info(const fpho_patch *p, int32_t *n_bands, int32_t *n_sources,
 int fpho_patch_set_launch(fpho_patch *p, const fpho_launch *launch);
 /* Precision of the pixel pass and reductions (FPHO_PREC_*). */
 int fpho_patch_set_precision(fpho_patch *p, int mode);
+/* Start (on != 0) or stop writing residual = data - model at every
+ * evaluation.  Stopping keeps the residual buffer, so a later
+ * fpho_patch_swap_data_residual still works; starting allocates it if the
+ * patch was created without one.  (forcepho sets patch.return_residual on
+ * the host, forcepho/patches/device_patch.py:288-293.) */
+int fpho_patch_set_residual_writes(fpho_patch *p, int on);
+/* Read (and with clear != 0, reset) the status word of the patch's stream-
+ * ordered evaluations (FPHO_STATUS_* bits); waits for the device first.
+ * The word is reset whenever the patch's plan is rebuilt. */
+int fpho_patch_status(fpho_patch *p, uint32_t *status, int clear);
 int fpho_patch_free(fpho_patch *p);
 
 /* Synchronous drop-in: proposal (host, n_sources entries) in, chi2[n_bands]
@@ -164,15 +196,41 @@ int fpho_patch_free(fpho_patch *p);
 int fpho_evaluate(fpho_patch *p, const fpho_source *proposal_host,
                   double *chi2_host, float *dchi2_host);
 /* Stream-ordered: all pointers are device pointers; dchi2 is double
- * [n_bands][n_sources][7].  No host synchronisation, no allocation. */
+ * [n_bands][n_sources][7].  No host synchronisation.  It allocates only
+ * when the patch changed since its last evaluation (new metadata, a swap,
+ * a launch or residual-write change): then its plan is rebuilt first, or,
+ * if `stream` is capturing a graph, FPHO_E_STATE is returned instead.
+ * Invalid proposal values set bits of the status word (fpho_patch_status). */
 int fpho_evaluate_async(fpho_patch *p, const fpho_source *proposal_dev,
                         double *chi2_dev, double *dchi2_dev, void *stream);
+
+/* The reference's output layout, for callers written against
+ * EvaluateProposal(patch, proposal, pchi2, pdchi2_dp)
+ * (forcepho/src/compute_gaussians_kernel.cu:212-214, 304-310): chi2[n_bands]
+ * and Response[n_bands], entries [0, 7 n_sources) of each band written
+ * (the reference leaves the rest unwritten; here it is zeroed).  Patches
+ * with more than FPHO_MAXSOURCES sources return FPHO_E_LIMIT.
+ * fpho_evaluate_response: host pointers, synchronous, as fpho_evaluate.
+ * fpho_evaluate_response_async: device pointers, stream ordered, allocation
+ * rules of fpho_evaluate_async. */
+int fpho_evaluate_response(fpho_patch *p, const fpho_source *proposal_host,
+                           double *chi2_host, fpho_response *response_host);
+int fpho_evaluate_response_async(fpho_patch *p, const fpho_source *proposal_dev,
+                                 double *chi2_dev, fpho_response *response_dev,
+                                 void *stream);
 
 /* ------------------------------------------------------------------ batch */
 /* Many patches evaluated by one set of launches: convert, one pixel kernel
  * (a grid launch or a persistent queue) over (patch, band, exposure,
- * pixel-block) tiles sorted by cost, one fixed-order FP64 reduction.  Patches stay owned by the caller and
- * must outlive the batch. */
+ * pixel-block) tiles sorted by cost, one fixed-order FP64 reduction.
+ * Patches stay owned by the caller and must outlive the batch.  The batch
+ * keeps the layout (n_bands, n_sources of every patch) it was created
+ * with: once a member's source count changes (fpho_patch_set_meta), every
+ * batch call returns FPHO_E_STATE, since the caller's buffers were sized by
+ * fpho_batch_layout.  Other member changes (new metadata of the same
+ * shape, a swap, residual writes) rebuild the plan at the next call,
+ * which allocates; while the call's stream is capturing a graph they
+ * return FPHO_E_STATE instead. */
 int fpho_batch_create(fpho_patch *const *patches, int n_patches,
                       fpho_batch **out);
 /* Offsets of each patch's slice in the concatenated arrays:
@@ -192,23 +250,12 @@ int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *proposals_dev,
  * and returns milliseconds of convert, pixel pass and reduce.  Timing adds
  * event records to the stream; leave it off when capturing graphs. */
 int fpho_batch_set_timing(fpho_batch *b, int on);
-/* Measurement: number of (pixel, image Gaussian) pairs inside the cutoff
- * (exp argument <= 36) for the given proposals (device), counted with the
- * pixel pass's own FP32 test.  Synchronous; allocates a counter. */
-int fpho_batch_count_pairs(fpho_batch *b, const fpho_source *proposals_dev,
-                           int64_t *n_in);
 int fpho_batch_last_timing(fpho_batch *b, float *ms3);
-/* Measurement only, no counterpart in forcepho: VALU issue calibration of
- * the current device.  kind 0 times independent FP32 FMA chains, kind 1 FMA
- * chains interleaved with v_exp_f32, kind 2 FMA chains in the accumulate
- * (v_fmac) encoding, kind 3 v_exp_f32 chains alone, kinds 4 and 5 FP32
- * multiply and add chains, kind 6 FMA chains with vector-register operands
- * only, at 8 waves per SIMD on every CU.
- * Returns the kernel time (ms, last of three runs), the wave64 VALU
- * instructions it issued and the device's shader clock (kHz), from which
- * instructions per SIMD per cycle follow.  Synchronous; allocates. */
-int fpho_issue_calibration(int kind, double *ms, double *wave_instructions,
-                           int *clock_khz);
+/* Status word of the batch's stream-ordered calls (FPHO_STATUS_* bits, from
+ * the proposal and conversion kernels); waits for the device first, resets
+ * the word with clear != 0 (a host call: do not make it inside a capture).
+ * The word is reset whenever the batch's plan is rebuilt. */
+int fpho_batch_status(fpho_batch *b, uint32_t *status, int clear);
 int fpho_batch_free(fpho_batch *b);
 
 /* ------------------------------------------ device proposals and gradients */

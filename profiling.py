@@ -7,13 +7,13 @@ device copy reads FETCH_SIZE = 0.5 GiB and WRITE_SIZE = 1 GiB.
 
 Usage (each ``<dir>`` is a rocprofv3 ``-d`` output directory):
 
-    python -m forcepho_amd.profiling stats <dir>
-    python -m forcepho_amd.profiling counters <dir> [--min-ms 10]
-    python -m forcepho_amd.profiling traffic --fetch <dir> --write <dir> \\
+    python profiling.py stats <dir>
+    python profiling.py counters <dir> [--min-ms 10]
+    python profiling.py traffic --fetch <dir> --write <dir> \\
         --cal-fetch <dir> --cal-write <dir> --launch K,W,P,G --command "..."
 
 ``traffic`` prints the summary that ``bench.py`` reads for ``roofline.traffic``.
-It carries a digest of the kernel sources' code (``csrc/kernels.hip`` and
+It carries a digest of the kernel sources' code (``forcepho_amd/csrc/kernels.hip`` and
 the two headers it compiles with), so a summary taken on other kernel code
 is recognised as stale.
 """
@@ -28,10 +28,11 @@ import sqlite3
 import statistics
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-KERNEL_SOURCE = os.path.join(HERE, "csrc", "kernels.hip")
+CSRC = os.path.join(HERE, "forcepho_amd", "csrc")
+KERNEL_SOURCE = os.path.join(CSRC, "kernels.hip")
 # the sources whose code fixes the pixel-pass machine code
-KERNEL_SOURCES = [KERNEL_SOURCE, os.path.join(HERE, "csrc", "kernels.h"),
-                  os.path.join(HERE, "csrc", "fpho_internal.h")]
+KERNEL_SOURCES = [KERNEL_SOURCE, os.path.join(CSRC, "kernels.h"),
+                  os.path.join(CSRC, "fpho_internal.h")]
 COPY_BYTES = 1 << 30   # bench.py --copy-calibration copies 1 GiB per dispatch
 
 __all__ = ["kernel_source_digest", "kernel_stats", "counter_medians", "read_summary"]

@@ -1,6 +1,7 @@
 #!/bin/bash
 # Convert the Sersic-mixture amplitude table shipped with the forcepho tests
-# (tests/data/sersic_splinedata_large.h5) into sersic_splinedata_large.npz.
+# (tests/data/sersic_splinedata_large.h5) into the package's data file
+# forcepho_amd/data/sersic_splinedata_large.npz.
 # h5py is not available in this image, so the datasets are dumped as raw
 # little-endian binary with h5dump and reassembled with numpy.
 # Usage: extract_splinedata.sh [path/to/sersic_splinedata_large.h5]
@@ -13,7 +14,7 @@ trap 'rm -r "$TMP"' EXIT
 for d in amplitudes nsersic radii rh; do
   "$H5DUMP" -b LE -d "/$d" -o "$TMP/$d.bin" "$SRC" > /dev/null
 done
-python3 - "$TMP" "$HERE/sersic_splinedata_large.npz" <<'PY'
+python3 - "$TMP" "$HERE/../../forcepho_amd/data/sersic_splinedata_large.npz" <<'PY'
 import sys
 import numpy as np
 tmp, out = sys.argv[1], sys.argv[2]

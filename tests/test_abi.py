@@ -43,6 +43,7 @@ int main(void) {
          offsetof(fpho_patch_desc, n_sources), offsetof(fpho_patch_desc, psfgauss),
          offsetof(fpho_patch_desc, psfgauss_start));
   printf("%zu %zu\n", offsetof(fpho_source, fluxes), offsetof(fpho_source, damplitude_dnsersic));
+  printf("%zu\n", sizeof(fpho_response));
   return 0;
 }
 ''')
@@ -61,6 +62,7 @@ int main(void) {
     assert v[7] == _lib.PatchDesc.psfgauss_start.offset
     assert v[8] == limits.source_struct_dtype.fields["fluxes"][1]
     assert v[9] == limits.source_struct_dtype.fields["damplitude_dnsersic"][1]
+    assert v[10] == limits.response_struct_dtype.itemsize == 840
 
 
 def _tiny_desc(keep, **over):
@@ -145,3 +147,32 @@ def test_one_hip_runtime_whatever_the_import_order():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                          check=True).stdout.split()
     assert out[-1] == "1"
+
+
+def test_new_entry_points_reject_null_arguments():
+    """The compat Response output, residual-write switch and status words
+    fail with FPHO_E_ARG on null handles, before any device work."""
+    L = _lib.lib()
+    z = ctypes.c_void_p(0)
+    w = ctypes.c_uint32()
+    assert L.fpho_evaluate_response(z, z, None, z) == 1
+    assert L.fpho_evaluate_response_async(z, z, z, z, z) == 1
+    assert L.fpho_patch_set_residual_writes(z, 1) == 1
+    assert L.fpho_patch_status(z, ctypes.byref(w), 0) == 1
+    assert L.fpho_batch_status(z, ctypes.byref(w), 0) == 1
+    assert _lib.describe_status(0x12) == ["axis ratio q <= 0",
+                                          "non-finite parameter vector entry"]
+
+
+def test_measurement_library_is_separate():
+    """Issue calibration and the in-cutoff pair count live in the bench-only
+    measure/libfpho_measure.so, not in the product library or its header."""
+    import measure
+    m = measure.lib()
+    for name in ("fpho_measure_issue_calibration", "fpho_measure_batch_count_pairs",
+                 "fpho_measure_last_error"):
+        assert hasattr(m, name)
+    lib = _lib.lib()
+    for name in ("fpho_issue_calibration", "fpho_batch_count_pairs"):
+        assert not hasattr(lib, name)
+    assert "calibrat" not in open(HEADER).read()

@@ -2,10 +2,9 @@
 the instruction count its loop implies at a plausible rate (a 16-lane SIMD
 issues at most one wave64 FP32 instruction per 4 cycles without packed
 FP32); an unknown kind is rejected on the host."""
-import ctypes
-
 import pytest
 
+import measure
 from forcepho_amd import _lib
 
 pytestmark = pytest.mark.gpu
@@ -16,17 +15,9 @@ PER_WAVE = {0: 16 * 32768, 1: 16 * 8192, 2: 16 * 32768, 3: 16 * 8192, 4: 16 * 32
             5: 16 * 32768, 6: 16 * 32768}
 
 
-def _run(kind):
-    ms, wi, clk = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-    rc = _lib.lib().fpho_issue_calibration(kind, ctypes.byref(ms), ctypes.byref(wi),
-                                           ctypes.byref(clk))
-    return rc, ms.value, wi.value, clk.value
-
-
 @pytest.mark.parametrize("kind", [0, 1, 2, 3, 4, 5, 6])
 def test_issue_calibration_rates(hip, kind):
-    rc, ms, wi, clk = _run(kind)
-    assert rc == 0, _lib.lib().fpho_last_error()
+    ms, wi, clk = measure.issue_calibration(kind)
     cus = _lib.cu_count()
     waves = 8 * cus * 4
     assert wi == waves * PER_WAVE[kind]
@@ -36,6 +27,5 @@ def test_issue_calibration_rates(hip, kind):
 
 
 def test_issue_calibration_rejects_unknown_kind(hip):
-    rc, _, _, _ = _run(7)
-    assert rc != 0
-    assert b"kind" in _lib.lib().fpho_last_error()
+    with pytest.raises(_lib.FphoError, match="kind"):
+        measure.issue_calibration(7)

@@ -3,7 +3,7 @@ import sqlite3
 
 import pytest
 
-from forcepho_amd import profiling
+import profiling
 
 
 @pytest.fixture()
