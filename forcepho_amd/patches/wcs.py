@@ -19,7 +19,7 @@ without them.
 
 import numpy as np
 
-__all__ = ["AffineWCS", "TanWCS", "WCS", "scale_at_sky"]
+__all__ = ["AffineWCS", "TanWCS", "WCS", "scale_at_sky", "local_affine"]
 
 _R2D = 180.0 / np.pi
 
@@ -219,3 +219,23 @@ def scale_at_sky(sky, wcs, dpix=1.0, origin=1, make_approx=False):
         D_mat = np.linalg.inv(wcs.pixel_scale_matrix * 3600.0)
         CW_mat = (D_mat * 3600.0) @ W
     return CW_mat, D_mat
+
+
+def local_affine(wcs, ra, dec):
+    """:func:`scale_at_sky` at many sky positions at once: (CW, D), each
+    [n, 2, 2], for ``ra``, ``dec`` arrays of length n (degrees).  Without
+    distortion D is the same at every position and CW differs only by
+    cos(dec); with distortion each position takes the finite-difference
+    branch."""
+    ra = np.atleast_1d(np.asarray(ra, dtype=np.float64))
+    dec = np.atleast_1d(np.asarray(dec, dtype=np.float64))
+    n = len(ra)
+    if getattr(wcs, "has_distortion", False):
+        pairs = [scale_at_sky((ra[i], dec[i]), wcs) for i in range(n)]
+        return (np.array([p[0] for p in pairs]).reshape(n, 2, 2),
+                np.array([p[1] for p in pairs]).reshape(n, 2, 2))
+    D = np.linalg.inv(np.asarray(wcs.pixel_scale_matrix, dtype=np.float64) * 3600.0)
+    CW = np.empty((n, 2, 2))
+    CW[:, :, 0] = 3600.0 * D[None, :, 0] * np.cos(np.deg2rad(dec))[:, None]
+    CW[:, :, 1] = 3600.0 * D[None, :, 1]
+    return CW, np.broadcast_to(D, (n, 2, 2)).copy()

@@ -8,6 +8,7 @@ Contents
 * ``evaluate`` -- FP64 C restatement (fpho_oracle.c) of the reference
   likelihood and gradient: chi2 = sum (r^2 - d^2) ierr^2 per band and
   dchi2/d(flux, ra, dec, q, pa, nsersic, rh) per band and source.
+* ``render`` -- the same restatement's FP64 model only (builds test data).
 * ``evaluate_np`` -- the same quantities in vectorised numpy FP64, written
   from forcepho/slow/gaussmodel.py:63-147,273-476 and
   forcepho/slow/likelihood.py:219-252 (used for small cases and to check the
@@ -71,6 +72,9 @@ def load():
         h.oracle_evaluate.argtypes = [ctypes.POINTER(_OraclePatch), _fp, ctypes.c_int, _dp,
                                       _dp, _dp, _dp]
         h.oracle_evaluate.restype = ctypes.c_int
+        h.oracle_evaluate64.argtypes = [ctypes.POINTER(_OraclePatch), _dp, ctypes.c_int, _dp,
+                                        _dp, _dp, _dp]
+        h.oracle_evaluate64.restype = ctypes.c_int
         h.oracle_image_gaussians.argtypes = [ctypes.POINTER(_OraclePatch), _fp, ctypes.c_int,
                                              ctypes.c_int, _dp]
         h.oracle_image_gaussians.restype = ctypes.c_int
@@ -143,6 +147,21 @@ def evaluate(patch, proposal, cutoff=True, residual=False, model=False):
     if model:
         out.append(mod)
     return tuple(out)
+
+
+def render(patch, proposal, cutoff=True):
+    """FP64 model [npix] of the proposal over the packed pixels (no
+    gradient; the data array may hold anything)."""
+    a = _arrays(patch)
+    d = _desc(patch, a)
+    p = _proposal(proposal)
+    chi2 = np.zeros(len(a["band_N"]))
+    mod = np.zeros(len(a["data"]))
+    rc = load().oracle_evaluate(ctypes.byref(d), p.ctypes.data_as(_fp), int(bool(cutoff)),
+                                chi2.ctypes.data_as(_dp), None, None, mod.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError("oracle_evaluate failed")
+    return mod
 
 
 def image_gaussians(patch, proposal, exposure):
@@ -406,6 +425,46 @@ def stack_grad(dchi2):
         g[:, b] += dchi2[b, :, 0]
         g[:, nb:] += dchi2[b, :, 1:]
     return g.reshape(-1)
+
+
+def source_records64(sources, n_bands):
+    """[n_sources, 66] FP64 records with the field order of the Source struct
+    (ra, dec, q, pa, nsersic, rh, fluxes[30], amplitudes[10],
+    damplitude_drh[10], damplitude_dnsersic[10])."""
+    rec = np.zeros((len(sources), 66))
+    for k, s in enumerate(sources):
+        rec[k, 0], rec[k, 1], rec[k, 2], rec[k, 3] = s["ra"], s["dec"], s["q"], s["pa"]
+        rec[k, 6:6 + n_bands] = s["fluxes"]
+        na = len(s["mixture_amplitudes"])
+        rec[k, 36:36 + na] = s["mixture_amplitudes"]
+        rec[k, 46:46 + na] = s["damplitude_drh"]
+        rec[k, 56:56 + na] = s["damplitude_dnsersic"]
+    return rec
+
+
+def evaluate64(patch, sources, cutoff=True):
+    """The C restatement fed FP64 source records (``sources_from_theta``
+    dicts): chi2 [n_bands], dchi2 [n_bands, n_sources, 7]."""
+    a = _arrays(patch)
+    d = _desc(patch, a)
+    nb, ns = len(a["band_N"]), int(patch.n_sources)
+    rec = np.ascontiguousarray(source_records64(sources, nb))
+    chi2 = np.zeros(nb)
+    dchi2 = np.zeros((nb, ns, 7))
+    rc = load().oracle_evaluate64(ctypes.byref(d), rec.ctypes.data_as(_dp), int(bool(cutoff)),
+                                  chi2.ctypes.data_as(_dp), dchi2.ctypes.data_as(_dp), None,
+                                  None)
+    if rc != 0:
+        raise RuntimeError("oracle_evaluate64 failed")
+    return chi2, dchi2
+
+
+def lnlike_theta(patch, theta, cutoff=True):
+    """lnlike = -chi2/2 and d lnlike / d theta, all FP64 (scipy splines, FP64
+    source records, the C restatement)."""
+    nb = len(np.asarray(patch.band_N))
+    chi2, dchi2 = evaluate64(patch, sources_from_theta(theta, nb), cutoff=cutoff)
+    return -0.5 * chi2.sum(), -0.5 * stack_grad(dchi2)
 
 
 def lnlike_theta_np(patch, theta, cutoff=True):

@@ -71,7 +71,7 @@ static void mat_T(const double a[4], double o[4]) {
 }
 
 /* Image Gaussians of source s in exposure e of band b. */
-static void convert_source(const oracle_patch *p, const float *src, int e, int b, int s,
+static void convert_source(const oracle_patch *p, const double *src, int e, int b, int s,
                            gauss64 *out) {
     const int ns = p->n_sources;
     const int np = p->n_psf_per_source[b];
@@ -193,59 +193,137 @@ static int pixel_gauss(const gauss64 *g, double xp, double yp, int cutoff, doubl
     return 1;
 }
 
-/* chi2[n_bands], dchi2[n_bands][n_sources][7]; residual/model [npix] or NULL.
- * Returns 0 on success. */
-int oracle_evaluate(const oracle_patch *p, const float *proposal, int cutoff, double *chi2,
-                    double *dchi2, double *residual, double *model) {
+/* Pixels per work unit: exposures are cut into chunks of this many pixels,
+ * evaluated in parallel, and each band's sums are taken over its units in
+ * a fixed order (exposure, then chunk), so the result does not depend on
+ * the number of threads. */
+#define ORACLE_CHUNK 4096
+
+/* chi2[n_bands], dchi2[n_bands][n_sources][7] (or NULL: no gradient, e.g.
+ * to render a model); residual/model [npix] or NULL, for source records in
+ * FP64 (SRC_STRIDE doubles each, the fields of the Source struct).  Returns
+ * 0 on success. */
+int oracle_evaluate64(const oracle_patch *p, const double *proposal, int cutoff, double *chi2,
+                      double *dchi2, double *residual, double *model) {
     const int ns = p->n_sources;
+    const int ne = p->n_exp;
     int status = 0;
-    /* bands are independent: one band per thread */
-#pragma omp parallel for schedule(dynamic, 1) reduction(| : status)
-    for (int b = 0; b < p->n_bands; ++b) {
+    int *band_of = (int *)malloc(sizeof(int) * (size_t)(ne > 0 ? ne : 1));
+    int64_t *goff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ne + 1));
+    int64_t *uoff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ne + 1));
+    if (!band_of || !goff || !uoff) {
+        free(band_of);
+        free(goff);
+        free(uoff);
+        return 1;
+    }
+    for (int e = 0; e < ne; ++e) band_of[e] = -1;
+    for (int b = 0; b < p->n_bands; ++b)
+        for (int e = p->band_start[b]; e < p->band_start[b] + p->band_N[b]; ++e) band_of[e] = b;
+    goff[0] = 0;
+    uoff[0] = 0;
+    for (int e = 0; e < ne; ++e) {
+        const int b = band_of[e];
+        const int64_t ng = b < 0 ? 0 : (int64_t)ns * p->n_psf_per_source[b];
+        const int64_t nu = b < 0 ? 0 : (p->exposure_N[e] + ORACLE_CHUNK - 1) / ORACLE_CHUNK;
+        goff[e + 1] = goff[e] + ng;
+        uoff[e + 1] = uoff[e] + nu;
+    }
+    const int64_t n_units = uoff[ne];
+    const int64_t nd = (int64_t)ns * 7;
+    gauss64 *gs = (gauss64 *)malloc(sizeof(gauss64) * (size_t)(goff[ne] > 0 ? goff[ne] : 1));
+    int *unit_exp = (int *)malloc(sizeof(int) * (size_t)(n_units > 0 ? n_units : 1));
+    double *uc2 = (double *)calloc((size_t)(n_units > 0 ? n_units : 1), sizeof(double));
+    double *ud = dchi2 ? (double *)calloc((size_t)(n_units > 0 ? n_units : 1) * (size_t)nd,
+                                          sizeof(double))
+                       : NULL;
+    if (!gs || !unit_exp || !uc2 || (dchi2 && !ud)) {
+        free(band_of); free(goff); free(uoff); free(gs); free(unit_exp); free(uc2); free(ud);
+        return 1;
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int64_t u = uoff[e]; u < uoff[e + 1]; ++u) unit_exp[u] = e;
+
+    /* image Gaussians of every exposure */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int e = 0; e < ne; ++e) {
+        const int b = band_of[e];
+        if (b < 0) continue;
+        const int np = p->n_psf_per_source[b];
+        for (int s = 0; s < ns; ++s)
+            convert_source(p, proposal + (int64_t)SRC_STRIDE * s, e, b, s, gs + goff[e] + s * np);
+    }
+
+    /* pixel chunks */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t u = 0; u < n_units; ++u) {
+        const int e = unit_exp[u];
+        const int b = band_of[e];
         const int np = p->n_psf_per_source[b];
         const int ng = ns * np;
-        gauss64 *gs = (gauss64 *)malloc(sizeof(gauss64) * (size_t)ng);
-        if (!gs) {
-            status |= 1;
-            continue;
-        }
+        const gauss64 *g = gs + goff[e];
+        const int64_t i0 = p->exposure_start[e] + (u - uoff[e]) * ORACLE_CHUNK;
+        int64_t i1 = i0 + ORACLE_CHUNK;
+        if (i1 > (int64_t)p->exposure_start[e] + p->exposure_N[e])
+            i1 = (int64_t)p->exposure_start[e] + p->exposure_N[e];
         double c2 = 0.0;
-        double *db = dchi2 + (int64_t)b * ns * 7;
-        memset(db, 0, sizeof(double) * ns * 7);
-        for (int e = p->band_start[b]; e < p->band_start[b] + p->band_N[b]; ++e) {
-            for (int s = 0; s < ns; ++s)
-                convert_source(p, proposal + (int64_t)SRC_STRIDE * s, e, b, s, gs + s * np);
-            for (int64_t i = p->exposure_start[e]; i < p->exposure_start[e] + p->exposure_N[e];
-                 ++i) {
-                const double xp = p->xpix[i], yp = p->ypix[i];
-                const double d = p->data[i], ie = p->ierr[i];
-                double m = 0.0, C;
-                for (int k = 0; k < ng; ++k)
-                    if (pixel_gauss(gs + k, xp, yp, cutoff, &C, NULL)) m += C;
-                const double r = d - m;
-                if (residual) residual[i] = r;
-                if (model) model[i] = m;
-                c2 += (r * ie - d * ie) * (r * ie + d * ie);
-                const double w = -2.0 * r * ie * ie;
-                if (w == 0.0) continue;
-                for (int s = 0; s < ns; ++s) {
-                    double *ds = db + 7 * s;
-                    for (int k = s * np; k < (s + 1) * np; ++k) {
-                        double dC[6];
-                        if (!pixel_gauss(gs + k, xp, yp, cutoff, &C, dC)) continue;
-                        for (int j = 0; j < 7; ++j) {
-                            double acc = 0.0;
-                            for (int l = 0; l < 6; ++l) acc += gs[k].jac[j][l] * dC[l];
-                            ds[j] += w * acc;
-                        }
+        double *du = ud ? ud + u * nd : NULL;
+        for (int64_t i = i0; i < i1; ++i) {
+            const double xp = p->xpix[i], yp = p->ypix[i];
+            const double d = p->data[i], ie = p->ierr[i];
+            double m = 0.0, C;
+            for (int k = 0; k < ng; ++k)
+                if (pixel_gauss(g + k, xp, yp, cutoff, &C, NULL)) m += C;
+            const double r = d - m;
+            if (residual) residual[i] = r;
+            if (model) model[i] = m;
+            c2 += (r * ie - d * ie) * (r * ie + d * ie);
+            const double w = -2.0 * r * ie * ie;
+            if (!du || w == 0.0) continue;
+            for (int s = 0; s < ns; ++s) {
+                double *ds = du + 7 * s;
+                for (int k = s * np; k < (s + 1) * np; ++k) {
+                    double dC[6];
+                    if (!pixel_gauss(g + k, xp, yp, cutoff, &C, dC)) continue;
+                    for (int j = 0; j < 7; ++j) {
+                        double acc = 0.0;
+                        for (int l = 0; l < 6; ++l) acc += g[k].jac[j][l] * dC[l];
+                        ds[j] += w * acc;
                     }
                 }
             }
         }
-        chi2[b] = c2;
-        free(gs);
+        uc2[u] = c2;
     }
+
+    /* per band, in unit order */
+    for (int b = 0; b < p->n_bands; ++b) {
+        double c2 = 0.0;
+        double *db = dchi2 ? dchi2 + (int64_t)b * nd : NULL;
+        if (db) memset(db, 0, sizeof(double) * (size_t)nd);
+        for (int e = p->band_start[b]; e < p->band_start[b] + p->band_N[b]; ++e)
+            for (int64_t u = uoff[e]; u < uoff[e + 1]; ++u) {
+                c2 += uc2[u];
+                if (db)
+                    for (int64_t j = 0; j < nd; ++j) db[j] += ud[u * nd + j];
+            }
+        chi2[b] = c2;
+    }
+    free(band_of); free(goff); free(uoff); free(gs); free(unit_exp); free(uc2); free(ud);
     return status;
+}
+
+/* The same for float32 Source records (the struct the kernels read): each
+ * field is widened exactly to double. */
+int oracle_evaluate(const oracle_patch *p, const float *proposal, int cutoff, double *chi2,
+                    double *dchi2, double *residual, double *model) {
+    const int64_t n = (int64_t)SRC_STRIDE * p->n_sources;
+    double *src = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!src) return 1;
+    for (int64_t i = 0; i < n; ++i) src[i] = proposal[i];
+    const int rc = oracle_evaluate64(p, src, cutoff, chi2, dchi2, residual, model);
+    free(src);
+    return rc;
 }
 
 /* The image Gaussians (amp, x, y, fxx, fyy, fxy then the 7x6 Jacobian,
@@ -257,8 +335,11 @@ int oracle_image_gaussians(const oracle_patch *p, const float *proposal, int e, 
     const int np = p->n_psf_per_source[b];
     gauss64 *gs = (gauss64 *)malloc(sizeof(gauss64) * (size_t)ns * np);
     if (!gs) return 1;
-    for (int s = 0; s < ns; ++s)
-        convert_source(p, proposal + (int64_t)SRC_STRIDE * s, e, b, s, gs + s * np);
+    double src[SRC_STRIDE];
+    for (int s = 0; s < ns; ++s) {
+        for (int i = 0; i < SRC_STRIDE; ++i) src[i] = proposal[(int64_t)SRC_STRIDE * s + i];
+        convert_source(p, src, e, b, s, gs + s * np);
+    }
     for (int k = 0; k < ns * np; ++k) {
         double *o = out + (int64_t)48 * k;
         o[0] = gs[k].amp;

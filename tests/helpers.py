@@ -13,7 +13,7 @@ GRAD_ATOL_FRAC = 1e-6
 
 
 def oracle_render(patch, proposal):
-    return oracle.evaluate(patch, proposal, model=True)[2]
+    return oracle.render(patch, proposal)
 
 
 def make_config_patch(config, patch_id=0, return_residual=False):
