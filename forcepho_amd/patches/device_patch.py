@@ -20,88 +20,116 @@ from ..limits import psf_struct_dtype
 __all__ = ["DevicePatchMixin", "HIPPatchMixin"]
 
 
+def _blocks(catalog, size):
+    """Consecutive rows of ``catalog`` in blocks of at most ``size``."""
+    if catalog is None or len(catalog) == 0:
+        return []
+    return [catalog[i:i + size] for i in range(0, len(catalog), size)]
+
+
 class DevicePatchMixin:
-    """Device-independent workflow on top of the device primitives."""
+    """The fitting workflow of forcepho/patches/device_patch.py:23-257 on top
+    of the device primitives of a mix-in such as :class:`HIPPatchMixin`.
 
-    def prepare_model(self, active=None, fixed=None, bounds=None, maxactive=15,
-                      model_kwargs={}):
-        """Subtract fixed sources, then wrap the patch in a FastPosterior
-        (forcepho/patches/device_patch.py:28-72).  ``bounds`` is a pair of
-        (lower, upper) arrays for a BoundedTransform."""
-        from ..model import FastPosterior, Transform
-        proposer, scene = self.subtract_fixed(fixed, active=active, maxactive=maxactive)
-        self.return_residual = False
-        q = scene.get_all_source_params().copy()
-        if bounds is None:
-            model = FastPosterior(proposer, scene=scene, patch=self,
-                                  transform=Transform(len(q)), **model_kwargs)
-        else:
-            lo, hi = bounds
-            model = FastPosterior(proposer, scene=scene, patch=self, lower=lo, upper=hi,
-                                  **model_kwargs)
-        return model, q
+    Every operation is a sequence of *scene loads* (pack a scene's metadata
+    and hand it to the device) and *evaluations*:
 
-    def subtract_fixed(self, fixed, active=None, big=None, maxactive=15, swap_local=False):
-        """Subtract blocks of big and fixed sources from the device-side data
-        (forcepho/patches/device_patch.py:113-183).  After the call the
-        device data hold the residual of all subtracted sources and the
-        device metadata describe the last scene (``active`` if given)."""
-        scenes = []
-        if big is not None:
-            inds = np.arange(maxactive, len(big), maxactive)
-            scenes.extend([self.set_scene(block) for block in np.array_split(big, inds)])
-        if fixed is not None:
-            inds = np.arange(maxactive, len(fixed), maxactive)
-            scenes.extend([self.set_scene(block) for block in np.array_split(fixed, inds)])
-        if active is not None:
-            scenes.append(self.set_scene(active))
-        else:
-            assert swap_local
-            scenes.append(scenes[-1])
-        assert len(scenes) > 0, "No scenes to prepare!"
-        proposer = self.get_proposer()
-        self.return_residual = True
-        self.pack_meta(scenes[0])
-        self.send_to_device()
-        for i, scene in enumerate(scenes[1:]):
-            proposal = scenes[i].get_proposal()
-            proposer.evaluate_proposal(proposal, patch=self)
-            self.pack_meta(scene)
-            self.swap_on_device()
-        return proposer, scenes[-1]
+    * ``subtract_fixed`` -- upload the pixels once, then for each block of
+      big or fixed sources: load it, evaluate (the device writes data -
+      model into the residual buffer) and swap data and residual, so the
+      device data lose that block; finally load the active scene;
+    * ``design_matrix`` -- per active source, load the source alone with
+      unit fluxes and read its model back (the model is linear in flux);
+    * ``render`` -- load a catalog and return minus the residual.
+    """
 
-    def design_matrix(self, active=None, proposer=None):
-        """Per-source unit-flux models per band and the fixed-subtracted data
-        (forcepho/patches/device_patch.py:185-238)."""
-        y = self.retrieve_array("data")
-        ys = np.split(y, np.cumsum(self.band_N_pix[:-1]))
-        Xes = [np.zeros((len(active), n)) for n in self.band_N_pix]
-        scenes = [self.set_scene(np.atleast_1d(a)) for a in active]
-        proposer = proposer or self.get_proposer()
-        self.return_residual = True
-        for i, scene in enumerate(scenes):
-            self.pack_meta(scene)
-            self.replace_device_meta_ptrs()
-            self.send_patchstruct_to_device()
-            proposal = scene.get_proposal()
-            _, _, residual = proposer.evaluate_proposal(proposal, patch=self, unpack=False)
-            model = y - residual
-            split = np.split(model, np.cumsum(self.band_N_pix[:-1]))
-            for j, b in enumerate(self.bandlist):
-                Xes[j][i, :] = split[j] / active[i][b]
-        return Xes, ys
-
-    def render(self, catalog):
-        """Model images of ``catalog`` (minus sign of the residual against
-        zero data is the model; forcepho/patches/device_patch.py:240-254)."""
-        proposer = self.get_proposer()
-        scene = self.set_scene(catalog)
+    def _load_scene(self, scene):
+        """Pack the scene's metadata and make it the device's."""
         self.pack_meta(scene)
         self.replace_device_meta_ptrs()
         self.send_patchstruct_to_device()
-        proposal = scene.get_proposal()
+        return scene
+
+    def prepare_model(self, active=None, fixed=None, bounds=None, maxactive=15,
+                      model_kwargs={}):
+        """Subtract the fixed sources, then wrap the patch and the active
+        scene in a :class:`~forcepho_amd.model.FastPosterior`
+        (device_patch.py:28-72).  ``bounds`` = (lower, upper) parameter
+        arrays gives a bounded transform, else the identity.  Returns the
+        model and the active scene's parameter vector."""
+        from ..model import FastPosterior, Transform
+        proposer, scene = self.subtract_fixed(fixed, active=active, maxactive=maxactive)
+        self.return_residual = False
+        q = np.array(scene.get_all_source_params(), dtype=np.float64)
+        if bounds is None:
+            limits = {"transform": Transform(len(q))}
+        else:
+            limits = {"lower": bounds[0], "upper": bounds[1]}
+        model = FastPosterior(proposer, scene=scene, patch=self, **limits, **model_kwargs)
+        return model, q
+
+    def subtract_fixed(self, fixed, active=None, big=None, maxactive=15, swap_local=False):
+        """Remove blocks (at most ``maxactive`` sources each) of ``big`` and
+        then ``fixed`` sources from the device-side data
+        (device_patch.py:113-183).  Afterwards the device data are the
+        residual of every subtracted block and the device metadata describe
+        ``active`` -- or, without it (``swap_local`` required), the last
+        subtracted block.  Returns (proposer, that scene)."""
+        removed = [self.set_scene(b) for b in _blocks(big, maxactive) + _blocks(fixed, maxactive)]
+        if active is None and not swap_local:
+            raise ValueError("without active sources, pass swap_local=True")
+        if not removed and active is None:
+            raise ValueError("No scenes to prepare!")
+        proposer = self.get_proposer()
         self.return_residual = True
-        _, _, residuals = proposer.evaluate_proposal(proposal, patch=self, unpack=True)
+        # the pixels go up with the first scene's metadata
+        first = removed[0] if removed else self.set_scene(active)
+        self.pack_meta(first)
+        self.send_to_device()
+        for k, scene in enumerate(removed):
+            if k > 0:
+                self._load_scene(scene)
+            proposer.evaluate_proposal(scene.get_proposal(), patch=self)
+            self.swap_on_device()           # data := data - model(scene)
+        if active is None:
+            return proposer, removed[-1]
+        if removed:
+            return proposer, self._load_scene(self.set_scene(active))
+        return proposer, first
+
+    def design_matrix(self, active=None, proposer=None):
+        """Design matrices for linear flux fitting (device_patch.py:
+        185-238): per band, X [n_active, n_pix_band] whose row i is source
+        i's model at unit flux in that band, and the target data y (the
+        device data, fixed sources already subtracted), so that
+        y_band ~ fluxes_band . X_band.  Returns (Xes, ys)."""
+        y = self.retrieve_array("data")
+        cuts = np.cumsum(self.band_N_pix)[:-1]
+        ys = np.split(y, cuts)
+        proposer = proposer or self.get_proposer()
+        self.return_residual = True
+        active = np.atleast_1d(active)
+        Xes = [np.zeros((len(active), n)) for n in self.band_N_pix]
+        for i in range(len(active)):
+            unit = active[i:i + 1].copy()       # a row of its own, not a view
+            for b in self.bandlist:
+                unit[b] = 1.0
+            scene = self._load_scene(self.set_scene(unit))
+            residual = proposer.evaluate_proposal(scene.get_proposal(), patch=self,
+                                                  unpack=False)[2]
+            for j, model_b in enumerate(np.split(y - residual, cuts)):
+                Xes[j][i] = model_b
+        return Xes, ys
+
+    def render(self, catalog):
+        """Minus the residual of ``catalog``'s scene against the device data
+        (device_patch.py:240-254): the model images when the device data
+        are zero.  One array per exposure."""
+        proposer = self.get_proposer()
+        scene = self._load_scene(self.set_scene(catalog))
+        self.return_residual = True
+        residuals = proposer.evaluate_proposal(scene.get_proposal(), patch=self,
+                                               unpack=True)[2]
         return [-r for r in residuals]
 
 
@@ -171,6 +199,7 @@ class HIPPatchMixin(DevicePatchMixin):
                                                 ctypes.byref(handle)))
         self._desc_keep = None
         self.device_patch = handle
+        self._device_npix = int(self.npix)
         self.device_ptrs = {"residual": bool(self.return_residual)}
         self._has_residual = bool(self.return_residual)
         self._dirty_data = False

@@ -33,7 +33,7 @@ from ..sources import Scene
 from .wcs import AffineWCS, local_affine, scale_at_sky
 
 __all__ = ["PatchBase", "Patch", "AffineWCS", "scale_at_sky",
-           "superpixel_order", "pack_exposures"]
+           "superpixel_order", "pack_exposures", "layout_exposures"]
 
 
 # --------------------------------------------------------------- pixels
@@ -68,39 +68,83 @@ def superpixel_order(nx, ny, sx, sy):
     return _layout(int(nx), int(ny), int(sx), int(sy))[0].copy()
 
 
-def pack_exposures(images, ierrs, bands, superpixel=(8, 8), warp_size=64,
-                   dtype=np.float32):
-    """Pack exposure images (each [nx, ny], pixel (i, j) at x = i, y = j)
-    into the device pixel arrays.  ``bands`` gives each exposure's band
-    index and must be non-decreasing.  Returns a dict of the arrays, plus
-    ``band_N_pix`` (padded pixels per band), ``original_shapes`` and
-    ``original_sizes``."""
+def _index_arrays(sizes, bands):
+    """exposure_start/exposure_N (int32), band_start/band_N (int16) and
+    band_N_pix (int32) for exposures of ``sizes`` pixels in ``bands``."""
+    sizes = np.asarray(sizes, dtype=np.int64)
     bands = np.asarray(bands, dtype=np.int64)
     if np.any(np.diff(bands) < 0):
         raise ValueError("Exposures must be sorted by band")
-    shapes = np.array([np.shape(im) for im in images], dtype=np.int64).reshape(-1, 2)
+    n_bands = int(bands.max()) + 1 if len(bands) else 0
+    band_N = np.bincount(bands, minlength=n_bands)
+    return {"exposure_start": np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32),
+            "exposure_N": sizes.astype(np.int32),
+            "band_N": band_N.astype(np.int16),
+            "band_start": np.concatenate([[0], np.cumsum(band_N)[:-1]]).astype(np.int16),
+            "band_N_pix": np.bincount(bands, weights=sizes,
+                                      minlength=n_bands).astype(np.int32)}
+
+
+def layout_exposures(chunks, bands, dtype=np.float32):
+    """Concatenate per-exposure pixel lists into the device arrays.
+
+    ``chunks`` holds one (flux, ierr, x, y) tuple of 1-D arrays per
+    exposure, already in the order and padding wanted; ``bands`` gives each
+    exposure's band index, non-decreasing.  Returns ``data, ierr, xpix,
+    ypix``, ``exposure_start/exposure_N`` (int32), ``band_start/band_N``
+    (int16, exposures per band) and ``band_N_pix`` (int32, pixels per
+    band)."""
+    sizes = [np.size(c[0]) for c in chunks]
+    out = _index_arrays(sizes, bands)
+    total = int(np.sum(sizes))
+    for k, name in enumerate(("data", "ierr", "xpix", "ypix")):
+        a = np.empty(total, dtype)
+        for c, s0, n in zip(chunks, out["exposure_start"], sizes):
+            a[s0:s0 + n] = np.asarray(c[k]).reshape(-1)
+        out[name] = a
+    return out
+
+
+def pack_exposures(images, ierrs, bands, superpixel=(8, 8), warp_size=64,
+                   dtype=np.float32, shapes=None):
+    """Pack exposure images (each [nx, ny], pixel (i, j) at x = i, y = j)
+    into the device pixel arrays: each exposure in superpixel order, padded
+    to a multiple of ``warp_size`` with flux = ierr = 0 and x = y = -1.
+    ``bands`` gives each exposure's band index and must be non-decreasing.
+    An image or ierr may be a scalar, a uniform image whose shape is then
+    ``shapes[e]``.  Returns the arrays of :func:`layout_exposures` plus
+    ``original_shapes`` and ``original_sizes``.  Every output element is
+    written once: the superpixel gather goes straight into its slice."""
+    def shape_of(e):
+        for a in (images[e], ierrs[e]):
+            if np.ndim(a) == 2:
+                return np.shape(a)
+        return tuple(shapes[e])
+    shapes = np.array([shape_of(e) for e in range(len(images))],
+                      dtype=np.int64).reshape(-1, 2)
     sizes = shapes.prod(axis=1)
     padded = -(-sizes // warp_size) * warp_size
-    starts = np.concatenate([[0], np.cumsum(padded)[:-1]])
+    out = _index_arrays(padded, bands)
     total = int(padded.sum())
-    n_bands = int(bands.max()) + 1
-    out = {"data": np.zeros(total, dtype), "ierr": np.zeros(total, dtype),
-           "xpix": np.full(total, -1, dtype), "ypix": np.full(total, -1, dtype)}
-    for e in range(len(images)):
+    data, ierr = np.empty(total, dtype), np.empty(total, dtype)
+    xpix, ypix = np.empty(total, dtype), np.empty(total, dtype)
+    for e, (s0, n, n_pad) in enumerate(zip(out["exposure_start"], sizes, padded)):
         order, x, y = _layout(int(shapes[e, 0]), int(shapes[e, 1]), *superpixel)
-        s = slice(starts[e], starts[e] + sizes[e])
-        out["data"][s] = np.asarray(images[e]).reshape(-1)[order]
-        out["ierr"][s] = np.asarray(ierrs[e]).reshape(-1)[order]
-        out["xpix"][s] = x
-        out["ypix"][s] = y
-    band_N = np.bincount(bands, minlength=n_bands)
-    out["exposure_start"] = starts.astype(np.int32)
-    out["exposure_N"] = padded.astype(np.int32)
-    out["band_N"] = band_N.astype(np.int16)
-    out["band_start"] = np.concatenate([[0], np.cumsum(band_N)[:-1]]).astype(np.int16)
-    out["band_N_pix"] = np.bincount(bands, weights=padded, minlength=n_bands).astype(np.int32)
-    out["original_shapes"] = shapes
-    out["original_sizes"] = sizes
+        sl, pad = slice(int(s0), int(s0 + n)), slice(int(s0 + n), int(s0 + n_pad))
+        for dst, src in ((data, images[e]), (ierr, ierrs[e])):
+            if np.ndim(src) == 0:
+                dst[sl] = src
+            else:
+                src = np.asarray(src).reshape(-1)
+                if src.dtype == dtype:
+                    np.take(src, order, out=dst[sl])
+                else:
+                    dst[sl] = src[order]
+            dst[pad] = 0
+        xpix[sl], ypix[sl] = x, y
+        xpix[pad] = ypix[pad] = -1
+    out.update(data=data, ierr=ierr, xpix=xpix, ypix=ypix, original_shapes=shapes,
+               original_sizes=sizes)
     return out
 
 
@@ -163,8 +207,13 @@ class PatchBase:
 
     @property
     def npix(self):
+        """Pixels of the patch (padding included): the length of the host
+        pixel arrays, or, once they were dropped after an upload, of the
+        device copy."""
         x = getattr(self, "xpix", None)
-        return 0 if x is None else int(np.shape(x)[0])
+        if x is not None:
+            return int(np.shape(x)[0])
+        return int(getattr(self, "_device_npix", 0))
 
     @property
     def size(self):
@@ -235,11 +284,13 @@ class Patch(PatchBase):
         self.patch_reference_coordinates = np.zeros(2)
 
     # -- pixels
-    def pack_pix(self, images, ierrs, bands, superpixel=(8, 8), warp_size=64):
+    def pack_pix(self, images, ierrs, bands, superpixel=(8, 8), warp_size=64, shapes=None):
         """Pack exposure images (each [nx, ny]) in superpixel order;
-        ``bands`` is each exposure's band index."""
+        ``bands`` is each exposure's band index (scalar images and ierrs
+        are uniform, of ``shapes[e]``)."""
         self.__dict__.update(pack_exposures(images, ierrs, bands, superpixel=superpixel,
-                                            warp_size=warp_size, dtype=self.pix_dtype))
+                                            warp_size=warp_size, dtype=self.pix_dtype,
+                                            shapes=shapes))
         self.n_exp = len(images)
         self.n_bands = len(self.band_N)
         self._dirty_data = True

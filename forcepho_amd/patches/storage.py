@@ -176,6 +176,42 @@ class _Dataset:
 
 
 # ----------------------------------------------------------------- pixels
+def clean_exposure(im, ierr, bkg=None, mask=None, bitmask=None):
+    """(flux, ierr) in FP64 of one exposure ready for the store.
+
+    flux = im - bkg.  A pixel is bad where flux or ierr is not finite, where
+    ierr < 0, or where ``mask`` flags it (with ``bitmask``: where any of
+    those bits is set in the integer mask, otherwise where the mask is
+    true).  Bad pixels get ierr = 0 and, as flux, the background that was
+    subtracted there (0 where it is not finite), so that they still sample
+    the background (storage.py:170-189)."""
+    im = np.asarray(im, dtype=np.float64)
+    ierr = np.asarray(ierr, dtype=np.float64)
+    sky = np.zeros_like(im) if bkg is None else np.asarray(bkg, dtype=np.float64)
+    flux = im - sky
+    with np.errstate(invalid="ignore"):
+        bad = ~(np.isfinite(flux) & np.isfinite(ierr) & (ierr >= 0))
+    if mask is not None:
+        flagged = np.asarray(mask)
+        flagged = (np.bitwise_and(flagged, bitmask) != 0) if bitmask else flagged.astype(bool)
+        bad |= flagged
+    fill = np.where(np.isfinite(sky), sky, 0.0)
+    return np.where(bad, fill, flux), np.where(bad, 0.0, ierr)
+
+
+def _exposure_attrs(fluxconv, units, bitmask, names):
+    """Attributes of a stored exposure: calibration, the bitmask applied and
+    the file names (a string, or the string fields of a named tuple)."""
+    attrs = {"counts_to_flux": fluxconv, "flux_units": units}
+    if bitmask:
+        attrs["bitmask_applied"] = int(bitmask)
+    if isinstance(names, str):
+        attrs["image_name"] = names
+    elif names is not None and hasattr(names, "_asdict"):
+        attrs.update({k: v for k, v in names._asdict().items() if isinstance(v, str)})
+    return attrs
+
+
 class PixelStore:
     """Exposure pixels in superpixel order (storage.py:60-299).
 
@@ -254,59 +290,30 @@ class PixelStore:
         return 1.0, "counts"
 
     def add_exposure(self, imset, bitmask=None, do_fluxcal=False):
-        """Store one exposure (storage.py:148-232): subtract the background,
-        mask non-finite or negative-ierr pixels (and, with ``bitmask``, mask
-        pixels with any of those bits set) to ierr = 0 with the subtracted
-        background as their flux, optionally flux-calibrate, superpixelize
-        and write ``<band>/<expID>/data``."""
-        hdr = imset.hdr
-        band, expID = imset.band, imset.expID
-        im = np.array(imset.im, dtype=np.float64)
-        ierr = np.array(imset.ierr, dtype=np.float64)
-        if imset.bkg is not None:
-            bkg = np.asarray(imset.bkg, dtype=np.float64)
-            im = im - bkg
-        else:
-            bkg = np.zeros_like(im)
-        mask = ~(np.isfinite(ierr) & np.isfinite(im) & (ierr >= 0))
-        if imset.mask is not None:
-            pmask = np.asarray(imset.mask)
-            if bitmask:
-                pmask = np.bitwise_and(pmask, bitmask) != 0
-            mask = mask | pmask.astype(bool)
-        ierr[mask] = 0
-        im[mask] = 0
-        gb = np.isfinite(bkg)
-        im[mask & gb] = bkg[mask & gb]
+        """Store one exposure as ``<band>/<expID>/data`` (storage.py:148-232):
+        background-subtracted, bad pixels given ierr = 0 (see
+        :func:`clean_exposure`), optionally flux-calibrated, in superpixel
+        order, with the calibration and the image names as attributes.  An
+        existing entry of the same exposure is replaced."""
+        flux, ierr = clean_exposure(imset.im, imset.ierr, bkg=imset.bkg, mask=imset.mask,
+                                    bitmask=bitmask)
+        fluxconv, units = self.flux_calibration(imset.hdr) if do_fluxcal else (1.0, "image")
         if do_fluxcal:
-            fluxconv, unitname = self.flux_calibration(hdr)
-            im *= fluxconv
-            ierr *= 1.0 / fluxconv
-        else:
-            fluxconv, unitname = 1.0, "image"
-        imsize = np.array(im.shape)
-        if np.any(imsize % self.super_pixel_size):
+            flux *= fluxconv
+            ierr /= fluxconv
+        shape = np.array(flux.shape)
+        if np.any(shape % self.super_pixel_size):
             raise ValueError("image size must be a multiple of the superpixel size")
-        if np.any(imsize != self._imsize()):
+        if np.any(shape != self._imsize()):
             raise ValueError("Image is not the expected size")
-        sp = self.superpixelize(im, ierr)
-        if not np.all(np.isfinite(sp)):
-            raise ValueError(f"There were non-finite pixels in exposure {expID}")
-        path = EXP_FMT.format(band, expID)
-        self._tree.delete(path)
-        dset = self._tree.create_dataset(path + "/data", sp)
-        a = dset.attrs
-        a["counts_to_flux"] = fluxconv
-        a["flux_units"] = unitname
-        if bitmask:
-            a["bitmask_applied"] = int(bitmask)
-        names = imset.names
-        if isinstance(names, str):
-            a["image_name"] = names
-        elif names is not None and hasattr(names, "_fields"):
-            for f, v in zip(names._fields, names):
-                if isinstance(v, str):
-                    a[f] = v
+        packed = self.superpixelize(flux, ierr)
+        if not np.isfinite(packed).all():
+            raise ValueError(f"There were non-finite pixels in exposure {imset.expID}")
+        group = EXP_FMT.format(imset.band, imset.expID)
+        self._tree.delete(group)
+        attrs = self._tree.create_dataset(group + "/data", packed).attrs
+        for key, value in _exposure_attrs(fluxconv, units, bitmask, imset.names).items():
+            attrs[key] = value
 
     @property
     def data(self):
@@ -318,68 +325,79 @@ class PixelStore:
 
 
 # --------------------------------------------------------------- metadata
+def _nested_map(tree, fn):
+    """{band: {expID: fn(value)}} of a two-level dict."""
+    return {band: {exp: fn(v) for exp, v in exps.items()} for band, exps in tree.items()}
+
+
+def _image_corners(hdr):
+    """Pixel (x, y) of the four corners of an NAXIS1 x NAXIS2 image, in
+    order round the image."""
+    nx, ny = float(hdr["NAXIS1"]), float(hdr["NAXIS2"])
+    return np.array([0.0, 0.0, nx, nx]), np.array([0.0, ny, ny, 0.0])
+
+
+def _s_region_polygon(hdr):
+    """(4, 2) ra, dec of the header's S_REGION POLYGON, or None."""
+    if "S_REGION" not in hdr:
+        return None
+    values = [float(v) for v in str(hdr["S_REGION"]).split()[2:]]
+    return np.array(values).reshape(4, 2)
+
+
 class MetaStore:
-    """FITS headers of the exposures, by band and expID, and their WCSs
-    (storage.py:302-522)."""
+    """FITS headers of the exposures by band and expID, with their WCSs
+    built on first use (storage.py:302-522).  On disk it is the reference's
+    JSON file of header strings, {band: {expID: header}}."""
 
     def __init__(self, metastorefile=None, load_all=False):
-        self.headers = {}
+        self.headers = {} if not metastorefile else self.read_from_file(metastorefile)
         self.wcs = {}
-        if metastorefile:
-            self.headers = self.read_from_file(metastorefile)
-            if load_all:
-                self.populate_wcs()
+        if load_all:
+            self.populate_wcs()
 
     def populate_wcs(self):
-        self.wcs = {}
-        for band, exps in self.headers.items():
-            self.wcs[band] = {e: WCS(h) for e, h in exps.items()}
+        self.wcs = _nested_map(self.headers, WCS)
 
     def get_wcs(self, band, expID):
-        """The WCS of one exposure, built on first use and cached."""
-        w = self.wcs.setdefault(band, {})
-        if expID not in w:
-            w[expID] = WCS(self.headers[band][expID])
-        return w[expID]
+        """The WCS of one exposure, built from its header once."""
+        cache = self.wcs.setdefault(band, {})
+        if expID not in cache:
+            cache[expID] = WCS(self.headers[band][expID])
+        return cache[expID]
 
     def add_exposure(self, imset):
-        band, expID = imset.band, imset.expID
         hdr = imset.hdr if isinstance(imset.hdr, Header) else Header(imset.hdr)
-        self.headers.setdefault(band, {})[expID] = hdr
+        self.headers.setdefault(imset.band, {})[imset.expID] = hdr
 
     def write_to_file(self, filename):
-        """{band: {expID: header string}} as JSON (the reference format)."""
         assert ".json" in filename
-        out = {b: {e: h.tostring() for e, h in exps.items()} for b, exps in self.headers.items()}
         with open(filename, "w") as f:
-            json.dump(out, f)
+            json.dump(_nested_map(self.headers, lambda h: h.tostring()), f)
 
     def read_from_file(self, filename):
         with open(filename) as f:
-            s = json.load(f)
-        return {b: {e: Header.fromstring(h) for e, h in exps.items()} for b, exps in s.items()}
+            return _nested_map(json.load(f), Header.fromstring)
+
+    def covers(self, band, expID, sky):
+        """Does the exposure's footprint contain sky = (ra, dec)?  The
+        S_REGION polygon, when the header has one, rules exposures out
+        cheaply; then the sky polygon of the image corners decides (it
+        avoids WCS errors far from the image)."""
+        hdr = self.headers[band][expID]
+        point = np.atleast_2d(sky)
+        rough = _s_region_polygon(hdr)
+        if rough is not None and not polygon_contains(rough, point)[0]:
+            return False
+        ra, dec = self.get_wcs(band, expID).pixel_to_world_values(*_image_corners(hdr))
+        return bool(polygon_contains(np.stack([ra, dec], axis=-1), point)[0])
 
     def find_exposures(self, sky, bandlist):
-        """(expIDs, bands) of the exposures in ``bandlist`` whose image
-        footprint contains ``sky`` = (ra, dec) (storage.py:415-476): a rough
-        S_REGION polygon test when the header has one, then the polygon of
-        the image corners through the WCS."""
-        epaths, bands = [], []
-        for band in bandlist:
-            for expID, hdr in self.headers.get(band, {}).items():
-                if "S_REGION" in hdr:
-                    poly = np.array([float(p) for p in str(hdr["S_REGION"]).split()[2:]])
-                    if not polygon_contains(poly.reshape(4, 2), np.atleast_2d(sky))[0]:
-                        continue
-                wcs = self.get_wcs(band, expID)
-                nx, ny = hdr["NAXIS1"], hdr["NAXIS2"]
-                px = np.array([0, 0, nx, nx], dtype=np.float64)
-                py = np.array([0, ny, ny, 0], dtype=np.float64)
-                bra, bdec = wcs.pixel_to_world_values(px, py)
-                if polygon_contains(np.stack([bra, bdec], axis=-1), np.atleast_2d(sky))[0]:
-                    epaths.append(expID)
-                    bands.append(band)
-        return epaths, bands
+        """(expIDs, bands) of the exposures in ``bandlist`` that cover sky
+        (storage.py:415-476)."""
+        hits = [(expID, band) for band in bandlist for expID in self.headers.get(band, {})
+                if self.covers(band, expID, sky)]
+        return [h[0] for h in hits], [h[1] for h in hits]
 
     @property
     def nexp(self):

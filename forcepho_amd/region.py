@@ -1,10 +1,19 @@
-"""Sky regions that select exposures and superpixels for a patch
+"""Sky regions that choose the exposures and superpixels of a patch
 (forcepho/region.py).
 
-A region answers two questions for the pixel packers: a celestial bounding
-box (to find exposures that may overlap it) and, given an exposure's WCS
-and the pixel corners of its superpixels, which superpixels it contains (a
-superpixel is in when any of its corners is).
+The pixel packers ask a region two things:
+
+* its celestial bounding polygon (``bounding_box``, corners SW, SE, NE, NW
+  as a (2, 4) array of ra, dec), to keep only exposures whose footprint
+  meets it (``overlaps``);
+* which superpixels of an exposure it takes (``contains``): given the pixel
+  coordinates of every superpixel's corners, (nsuper_x, nsuper_y, 4) arrays,
+  the ``np.where`` indices of the superpixels with at least one corner
+  inside.
+
+Each region here states a single per-corner predicate, ``corner_mask``; the
+base class reduces it over the corners.  Both shipped regions have RA/Dec
+aligned extents, from which their bounding polygon follows.
 """
 
 import numpy as np
@@ -15,31 +24,57 @@ __all__ = ["Region", "CircularRegion", "RectangularRegion",
            "polygons_overlap", "polygon_contains"]
 
 
-class Region:
-    """Base class (forcepho/region.py:16-62)."""
+def _aligned_box(ra_lo, ra_hi, dec_lo, dec_hi):
+    """(2, 4) ra, dec of an RA/Dec aligned box, corners SW, SE, NE, NW."""
+    return np.array([[ra_lo, ra_hi, ra_hi, ra_lo],
+                     [dec_lo, dec_lo, dec_hi, dec_hi]], dtype=np.float64)
 
-    def contains(self, x, y, wcs):
-        return np.zeros(len(x), dtype=bool)
+
+class Region:
+    """A patch footprint on the sky (forcepho/region.py:16-62)."""
+
+    def __init__(self, *args, **kwargs):
+        pass
+
+    def corner_mask(self, xcorners, ycorners, wcs):
+        """Boolean array (the shape of the corner arrays): corner inside."""
+        return np.zeros(np.shape(xcorners), dtype=bool)
+
+    def contains(self, xcorners, ycorners, wcs):
+        """Indices of the superpixels with at least one corner inside."""
+        mask = self.corner_mask(np.asarray(xcorners), np.asarray(ycorners), wcs)
+        if mask.ndim == 1:
+            return mask
+        return np.nonzero(mask.any(axis=-1))
+
+    @property
+    def extent(self):
+        """(ra_lo, ra_hi, dec_lo, dec_hi) of the aligned bounding box."""
+        return None
 
     @property
     def bounding_box(self):
-        return None
+        ext = self.extent
+        return None if ext is None else _aligned_box(*ext)
 
     def overlaps(self, hdr):
-        """Whether this region's bounding box meets the celestial bounding
-        box of an image with WCS and size from ``hdr``."""
+        """Does the bounding polygon meet the RA/Dec aligned box around an
+        image whose WCS and NAXIS1 x NAXIS2 size come from ``hdr``?"""
         wcs = WCS(hdr)
-        nx, ny = hdr["NAXIS1"], hdr["NAXIS2"]
-        px = np.array([0, nx, nx, 0], dtype=np.float64)
-        py = np.array([0, 0, ny, ny], dtype=np.float64)
-        ira, idec = wcs.pixel_to_world_values(px, py)
-        im_bb = RectangularRegion(ira.min(), ira.max(), idec.min(), idec.max())
-        return polygons_overlap(self, im_bb)
+        w, h = float(hdr["NAXIS1"]), float(hdr["NAXIS2"])
+        ra, dec = wcs.pixel_to_world_values(np.array([0.0, w, w, 0.0]),
+                                            np.array([0.0, 0.0, h, h]))
+        image = RectangularRegion(np.min(ra), np.max(ra), np.min(dec), np.max(dec))
+        return polygons_overlap(self, image)
 
 
 class CircularRegion(Region):
-    """A circle on the sky (forcepho/region.py:65-136): centre (ra, dec)
-    and radius, all in degrees."""
+    """A circle of ``radius`` degrees of arc about (ra, dec) in degrees
+    (forcepho/region.py:65-136).
+
+    In an exposure the circle becomes the pixel circle about the centre's
+    pixel position whose radius is the pixel distance from the centre to the
+    point ``radius`` north of it; a corner is inside when strictly closer."""
 
     def __init__(self, ra, dec, radius):
         self.ra = ra
@@ -47,29 +82,23 @@ class CircularRegion(Region):
         self.radius = radius
         self.shape = "CircularRegion"
 
-    def contains(self, xcorners, ycorners, wcs):
-        """Indices (2-tuple from ``np.where``) of the superpixels with a
-        corner inside the circle; ``xcorners``/``ycorners`` are
-        (nsuper_x, nsuper_y, 4) pixel coordinates.  The radius is measured
-        in pixels along the exposure's Dec direction at the centre."""
-        xc, yc = wcs.world_to_pixel_values(self.ra, self.dec)
-        xr, yr = wcs.world_to_pixel_values(self.ra, self.dec + self.radius)
-        r2 = (xc - xr) ** 2 + (yc - yr) ** 2
-        d2 = (xc - xcorners) ** 2 + (yc - ycorners) ** 2
-        return np.where(np.any(d2 < r2, axis=-1))
+    def corner_mask(self, xcorners, ycorners, wcs):
+        cx, cy = wcs.world_to_pixel_values(self.ra, self.dec)
+        nx, ny = wcs.world_to_pixel_values(self.ra, self.dec + self.radius)
+        r_pix2 = np.square(nx - cx) + np.square(ny - cy)
+        return np.square(xcorners - cx) + np.square(ycorners - cy) < r_pix2
 
     @property
-    def bounding_box(self):
-        """(2, 4) ra, dec of the corners (SW, SE, NE, NW) of the RA/Dec
-        aligned box circumscribing the circle."""
-        dra = self.radius / np.cos(np.deg2rad(self.dec))
-        ddec = self.radius
-        return np.array([(self.ra - dra, self.dec - ddec), (self.ra + dra, self.dec - ddec),
-                         (self.ra + dra, self.dec + ddec), (self.ra - dra, self.dec + ddec)]).T
+    def extent(self):
+        half_ra = self.radius / np.cos(np.deg2rad(self.dec))
+        return (self.ra - half_ra, self.ra + half_ra,
+                self.dec - self.radius, self.dec + self.radius)
 
 
 class RectangularRegion(Region):
-    """An RA/Dec aligned box (forcepho/region.py:139-211), degrees."""
+    """The RA/Dec aligned box ra_min < ra < ra_max, dec_min < dec < dec_max
+    in degrees (forcepho/region.py:139-211); a corner is inside when its sky
+    position is strictly inside."""
 
     def __init__(self, ra_min, ra_max, dec_min, dec_max):
         self.ra_min = ra_min
@@ -78,17 +107,15 @@ class RectangularRegion(Region):
         self.dec_max = dec_max
         self.shape = "RectangularRegion"
 
-    def contains(self, xcorners, ycorners, wcs):
-        """Indices of the superpixels with a corner strictly inside the box."""
+    def corner_mask(self, xcorners, ycorners, wcs):
         ra, dec = wcs.pixel_to_world_values(xcorners, ycorners)
-        valid = ((ra > self.ra_min) & (ra < self.ra_max) &
-                 (dec > self.dec_min) & (dec < self.dec_max))
-        return np.where(np.any(valid, axis=-1))
+        in_ra = (self.ra_min < ra) & (ra < self.ra_max)
+        in_dec = (self.dec_min < dec) & (dec < self.dec_max)
+        return in_ra & in_dec
 
     @property
-    def bounding_box(self):
-        return np.array([(self.ra_min, self.dec_min), (self.ra_max, self.dec_min),
-                         (self.ra_max, self.dec_max), (self.ra_min, self.dec_max)]).T
+    def extent(self):
+        return (self.ra_min, self.ra_max, self.dec_min, self.dec_max)
 
 
 def polygons_overlap(a, b):

@@ -192,12 +192,12 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
                                         spec.psf_elliptical)
             psfs.append(band_psf[b])
             bands.append(b)
-            ie = np.full((side, side), 1.0 / sigmas[b], dtype=np.float32)
+            ie = np.float32(1.0 / sigmas[b])
             if spec.mask_frac > 0:
-                mask = rng.random((side, side)) < spec.mask_frac
-                ie[mask] = 0.0
+                ie = np.full((side, side), ie, dtype=np.float32)
+                ie[rng.random((side, side)) < spec.mask_frac] = 0.0
             ierrs.append(ie)
-            images.append(np.zeros((side, side), dtype=np.float32))
+            images.append(np.float32(0.0))
 
     patch = HIPPatch(return_residual=return_residual)
     patch.bandlist = bandlist
@@ -205,7 +205,7 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
     patch.wcses = wcses
     patch.psfs = psfs
     patch.pack_pix(images, ierrs, bands, superpixel=spec.superpixel,
-                   warp_size=spec.warp_size)
+                   warp_size=spec.warp_size, shapes=[(side, side)] * len(images))
     if spec.pix_offset:
         patch.xpix = (patch.xpix + np.float32(spec.pix_offset)).astype(np.float32)
         patch.ypix = (patch.ypix + np.float32(spec.pix_offset)).astype(np.float32)
