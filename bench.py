@@ -163,28 +163,48 @@ def algorithmic_counts(specs):
     return pairs, npix, flops
 
 
-def build_patches(specs, log_every=64, config=4, keep_host=None):
+def _noise(patch, config, i):
+    """Seeded Gaussian noise of each band's sigma over the patch's real
+    pixels (padding stays 0), drawn on the device (the host's generator
+    would take minutes at the mosaic workload's 4.9e9 pixels) and copied to
+    a host array for upload."""
+    import torch
+    seed = int(np.random.SeedSequence([config, i, 2]).generate_state(1)[0])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    noise = torch.randn(patch.npix, generator=gen, device="cuda", dtype=torch.float32)
+    sigma = torch.zeros_like(noise)
+    band_of = np.repeat(np.arange(len(patch.band_N)), patch.band_N)
+    for e in range(len(patch.exposure_N)):
+        s0, n = int(patch.exposure_start[e]), int(patch.original_sizes[e])
+        sigma[s0:s0 + n] = float(np.float32(patch.sigma_band[band_of[e]]))
+    return (noise * sigma).cpu().numpy()
+
+
+def build_patches(specs, log_every=64, config=4, keep_host=None, prebuilt=None):
     """Upload every patch with data = noise, render the truth on the device
     with negated fluxes (the model is linear in flux) so that the residual
     buffer holds noise + model, and make it the data (swap_on_device);
     then stop residual writes, so that evaluations write nothing but their
     outputs.  Host pixel arrays are dropped after upload except for the
-    patches in ``keep_host`` (None keeps all)."""
+    patches in ``keep_host`` (None keeps all).  ``prebuilt`` maps patch
+    indices to (patch, scene, theta) built by the caller with host data;
+    those are uploaded as they are."""
     import ctypes
     from forcepho_amd import _lib, synth
     from forcepho_amd.limits import source_struct_dtype
+    prebuilt = prebuilt or {}
     out = []
     for i, spec in enumerate(specs):
+        if i in prebuilt:
+            patch, scene, theta = prebuilt[i]
+            patch.return_residual = False
+            patch.send_to_device()
+            out.append((patch, scene, theta))
+            continue
         patch, scene, theta = synth.make_patch(spec, config=config, patch_id=i,
                                                return_residual=True)
-        rng = synth.rng_for(config, i, 2)
-        noise = np.zeros(patch.npix, dtype=np.float32)
-        for e in range(len(patch.exposure_N)):
-            b = int(np.searchsorted(np.cumsum(patch.band_N), e, side="right"))
-            s0, n = patch.exposure_start[e], patch.original_sizes[e]
-            noise[s0:s0 + n] = rng.standard_normal(n, dtype=np.float32) * \
-                np.float32(patch.sigma_band[b])
-        patch.data = noise
+        patch.data = _noise(patch, config, i)
         patch.send_to_device()
         neg = scene.get_proposal().copy()
         neg["fluxes"] *= -1.0
@@ -199,8 +219,7 @@ def build_patches(specs, log_every=64, config=4, keep_host=None):
         patch.set_residual_writes(False)
         patch.data = None  # host copy no longer describes the device data
         if keep_host is not None and i not in keep_host:
-            patch.ierr = None   # xpix stays: it sizes the patch (npix)
-            patch.ypix = None
+            patch.ierr = patch.xpix = patch.ypix = None
         out.append((patch, scene, theta))
         if (i + 1) % log_every == 0:
             log(f"  built {i + 1}/{len(specs)} patches")

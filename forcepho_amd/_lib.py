@@ -34,7 +34,8 @@ EXPORTED = [
     "fpho_batch_evaluate_async", "fpho_batch_set_timing", "fpho_batch_last_timing",
     "fpho_batch_status", "fpho_batch_free",
     "fpho_spline_create", "fpho_spline_free",
-    "fpho_batch_theta_to_proposals_async", "fpho_batch_stack_grad_async",
+    "fpho_batch_theta_to_proposals_async", "fpho_batch_evaluate_theta_async",
+    "fpho_batch_stack_grad_async",
 ]
 
 ARRAY_IDS = {"data": 0, "ierr": 1, "xpix": 2, "ypix": 3, "residual": 4}
@@ -119,6 +120,7 @@ def _declare(lib):
                                 c_double_p, ctypes.c_double, ctypes.POINTER(P)], c_int),
         "fpho_spline_free": ([P], c_int),
         "fpho_batch_theta_to_proposals_async": ([P, P, P, P, P], c_int),
+        "fpho_batch_evaluate_theta_async": ([P, P, P, P, P, P, P], c_int),
         "fpho_batch_stack_grad_async": ([P, P, P, P, P, P], c_int),
     }
     for name, (args, res) in sig.items():

@@ -181,12 +181,23 @@ class PatchBatch:
             _stream_ptr(stream)))
         return self.proposals
 
+    def evaluate_theta(self, theta, splines, stream=None):
+        """chi2 and dchi2 of every patch at the parameter vector ``theta``
+        (float64 on the device): the image Gaussians are built from FP64
+        source parameters (the FP32 records are written to ``proposals``
+        too)."""
+        assert theta.dtype == torch.float64 and theta.is_cuda and theta.numel() == self.n_theta
+        theta = theta.contiguous()
+        _lib.check(_lib.lib().fpho_batch_evaluate_theta_async(
+            self.handle, splines.handle, _lib.dptr(theta), _lib.dptr(self.proposals),
+            _lib.dptr(self.chi2), _lib.dptr(self.dchi2), _stream_ptr(stream)))
+        return self.chi2, self.dchi2
+
     def lnlike_and_grad(self, theta, splines, stream=None):
         """lnlike [n_patches] = -chi2/2 per patch and its gradient [n_theta]
         with respect to theta (per source [flux_1..flux_NB, ra, dec, q, pa,
         nsersic, rh]); all on the device."""
-        self.theta_to_proposals(theta, splines, stream)
-        self.evaluate(stream=stream)
+        self.evaluate_theta(theta, splines, stream)
         _lib.check(_lib.lib().fpho_batch_stack_grad_async(
             self.handle, _lib.dptr(self.chi2), _lib.dptr(self.dchi2), _lib.dptr(self.lnlike),
             _lib.dptr(self.grad), _stream_ptr(stream)))

@@ -129,6 +129,19 @@ struct alignas(8) SrcDesc {
 };
 static_assert(sizeof(SrcDesc) == 32, "SrcDesc must be 32 bytes");
 
+// The fields of fpho_source in FP64: what the parameter-vector path hands
+// the conversion stage, so that positions and shapes reach it unrounded
+// (an FP32 ra/dec offset is quantised at ~3e-6 pixel, which moves the
+// gradient of a bright source's position by up to the stated tolerance).
+struct alignas(16) Source64 {
+    double ra, dec, q, pa, nsersic, rh;
+    double fluxes[FPHO_MAXBANDS];
+    double mixture_amplitudes[FPHO_MAXRADII];
+    double damplitude_drh[FPHO_MAXRADII];
+    double damplitude_dnsersic[FPHO_MAXRADII];
+};
+static_assert(sizeof(Source64) == 528, "Source64 must be 528 bytes");
+
 struct alignas(8) PatchOut {
     int64_t chi2_off;    // first band of this patch in chi2 output
     int32_t n_bands;

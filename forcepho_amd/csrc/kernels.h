@@ -46,6 +46,9 @@ struct SplineView {
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
                           ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
                           hipStream_t stream);
+hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const Source64 *props,
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
+                          hipStream_t stream);
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream);
 hipError_t launch_response(const double *dchi2, int32_t n_bands, int32_t n_src,
                            fpho_response *out, hipStream_t stream);
@@ -53,8 +56,8 @@ hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart
                          const double *cpart, double *chi2, double *dchi2, int fp64,
                          hipStream_t stream);
 hipError_t launch_theta(const SrcDesc *srcs, int64_t n_srcs, const double *theta,
-                        const SplineView &sp, fpho_source *props, uint32_t *status,
-                        hipStream_t stream);
+                        const SplineView &sp, fpho_source *props, Source64 *props64,
+                        uint32_t *status, hipStream_t stream);
 hipError_t launch_stack(const SrcDesc *srcs, int64_t n_srcs, const PatchOut *pouts,
                         int64_t n_patches, const double *chi2, const double *dchi2,
                         double *lnlike, double *grad, hipStream_t stream);

@@ -11,9 +11,9 @@
 //   reduce_kernel       fixed-order sum of tile partials (FP64 or FP32)
 //                       (replaces the shared-memory Accumulator,
 //                        compute_gaussians_kernel.cu:133-198)
-//   theta_kernel        parameter vector -> fpho_source structs, Sersic
-//                       amplitudes from bicubic B-splines in FP64
-//                       (forcepho/sources.py:607-694)
+//   theta_kernel        parameter vector -> source records (FP32 fpho_source
+//                       and/or FP64 Source64), Sersic amplitudes from
+//                       bicubic B-splines in FP64 (forcepho/sources.py:607-694)
 //   stack_kernel        lnlike = -chi2/2 and stacked gradient
 //                       (forcepho/model.py:331-332,370-387)
 #include <hip/hip_runtime.h>
@@ -31,9 +31,11 @@ namespace fpho {
 // is negligible next to the pixel pass, so it runs in double and is rounded
 // once into the FP32 record the pixel pass streams.
 // Returns the FPHO_STATUS_* bits of what it met (0 for a valid Gaussian).
-__device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e,
-                                                        const fpho_source &src, int s,
-                                                        int p, ImgGauss &g,
+// Src is fpho_source (the FP32 proposal record) or Source64 (the same fields
+// in FP64, from the parameter-vector path).
+template <typename Src>
+__device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e, const Src &src,
+                                                        int s, int p, ImgGauss &g,
                                                         ImgGauss64 &g64) {
     using M = M22<double>;
     const float *Dp = e.D + 4 * s;
@@ -147,7 +149,7 @@ __device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e,
           isfinite(flux) && isfinite(am) && isfinite(src.damplitude_dnsersic[bin]) &&
           isfinite(src.damplitude_drh[bin])))
         flags |= FPHO_STATUS_PROPOSAL_NONFINITE;
-    if (!(src.q > 0.0f)) flags |= FPHO_STATUS_Q_NONPOSITIVE;
+    if (!(src.q > 0)) flags |= FPHO_STATUS_Q_NONPOSITIVE;
     if (!(det(cov) > 0.0)) flags |= FPHO_STATUS_COVARIANCE;
     if (!(isfinite(amp) && isfinite(xc) && isfinite(yc) && isfinite(F.a) && isfinite(F.c) &&
           isfinite(F.d) && isfinite(dA_dq) && isfinite(dA_dpa)))
@@ -157,8 +159,9 @@ __device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e,
 
 // One workgroup per exposure; lanes stride over its n_src * n_psf Gaussians.
 // Invalid inputs set bits of *status (a vector atomic, taken only then).
+template <typename Src>
 __global__ __launch_bounds__(128) void convert_kernel(
-    const ExpDesc *__restrict__ exps, const fpho_source *__restrict__ props,
+    const ExpDesc *__restrict__ exps, const Src *__restrict__ props,
     ImgGauss *__restrict__ out, ImgGauss64 *__restrict__ out64, int4 *__restrict__ boxes,
     uint32_t *__restrict__ status) {
     const ExpDesc e = exps[blockIdx.x];
@@ -716,9 +719,12 @@ __device__ inline int knot_span(const double *t, int n_t, double x) {
     return lo;
 }
 
+// Writes the FP32 record to props and/or the FP64 record to props64 (either
+// may be null).
 __global__ __launch_bounds__(64) void theta_kernel(
     const SrcDesc *__restrict__ srcs, int32_t n_srcs, const double *__restrict__ theta,
-    SplineView sp, fpho_source *__restrict__ props, uint32_t *__restrict__ status) {
+    SplineView sp, fpho_source *__restrict__ props, Source64 *__restrict__ props64,
+    uint32_t *__restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_srcs) return;
     const SrcDesc sd = srcs[i];
@@ -728,23 +734,22 @@ __global__ __launch_bounds__(64) void theta_kernel(
         for (int j = 0; j < sd.n_bands + 6; ++j) finite = finite && isfinite(th[j]);
         if (!finite) atomicOr(status, FPHO_STATUS_THETA_NONFINITE);
     }
-    fpho_source o;
-    for (int b = 0; b < FPHO_MAXBANDS; ++b) o.fluxes[b] = 0.0f;
+    Source64 o;
+    for (int b = 0; b < FPHO_MAXBANDS; ++b) o.fluxes[b] = 0.0;
     for (int m = 0; m < FPHO_MAXRADII; ++m) {
-        o.mixture_amplitudes[m] = 0.0f;
-        o.damplitude_drh[m] = 0.0f;
-        o.damplitude_dnsersic[m] = 0.0f;
+        o.mixture_amplitudes[m] = 0.0;
+        o.damplitude_drh[m] = 0.0;
+        o.damplitude_dnsersic[m] = 0.0;
     }
     const int nb = sd.n_bands;
-    for (int b = 0; b < nb; ++b) o.fluxes[b] = (float)th[b];
-    const double ra = th[nb + 0], dec = th[nb + 1], q = th[nb + 2], pa = th[nb + 3];
-    const double n = th[nb + 4], rh = th[nb + 5];
-    o.ra = (float)ra;
-    o.dec = (float)dec;
-    o.q = (float)q;
-    o.pa = (float)pa;
-    o.nsersic = (float)n;
-    o.rh = (float)rh;
+    for (int b = 0; b < nb; ++b) o.fluxes[b] = th[b];
+    o.ra = th[nb + 0];
+    o.dec = th[nb + 1];
+    o.q = th[nb + 2];
+    o.pa = th[nb + 3];
+    o.nsersic = th[nb + 4];
+    o.rh = th[nb + 5];
+    const double n = o.nsersic, rh = o.rh;
 
     if (n < sp.nsersic_min) {
         // one-hot at the radius closest to rh (forcepho/sources.py:662-665)
@@ -757,9 +762,9 @@ __global__ __launch_bounds__(64) void theta_kernel(
                 best = m;
             }
         }
-        o.mixture_amplitudes[best] = 1.0f;
+        o.mixture_amplitudes[best] = 1.0;
     } else if (rh < 0.0) {
-        o.mixture_amplitudes[0] = 1.0f;  // forcepho/sources.py:666-668
+        o.mixture_amplitudes[0] = 1.0;  // forcepho/sources.py:666-668
     } else {
         const double xn = fmin(fmax(n, sp.tx[3]), sp.tx[sp.n_tx - 4]);
         const double xr = fmin(fmax(rh, sp.ty[3]), sp.ty[sp.n_ty - 4]);
@@ -784,12 +789,28 @@ __global__ __launch_bounds__(64) void theta_kernel(
                 vn += dNx[a] * rv;
                 vr += Nx[a] * rd;
             }
-            o.mixture_amplitudes[m] = (float)v;
-            o.damplitude_dnsersic[m] = (float)vn;
-            o.damplitude_drh[m] = (float)vr;
+            o.mixture_amplitudes[m] = v;
+            o.damplitude_dnsersic[m] = vn;
+            o.damplitude_drh[m] = vr;
         }
     }
-    props[sd.proposal] = o;
+    if (props64) props64[sd.proposal] = o;
+    if (props) {
+        fpho_source f;
+        f.ra = (float)o.ra;
+        f.dec = (float)o.dec;
+        f.q = (float)o.q;
+        f.pa = (float)o.pa;
+        f.nsersic = (float)o.nsersic;
+        f.rh = (float)o.rh;
+        for (int b = 0; b < FPHO_MAXBANDS; ++b) f.fluxes[b] = (float)o.fluxes[b];
+        for (int m = 0; m < FPHO_MAXRADII; ++m) {
+            f.mixture_amplitudes[m] = (float)o.mixture_amplitudes[m];
+            f.damplitude_drh[m] = (float)o.damplitude_drh[m];
+            f.damplitude_dnsersic[m] = (float)o.damplitude_dnsersic[m];
+        }
+        props[sd.proposal] = f;
+    }
 }
 
 // ------------------------------------------------------- gradient stacking
@@ -835,13 +856,26 @@ __global__ __launch_bounds__(256) void response_kernel(const double *__restrict_
 
 // ---------------------------------------------------------------- launchers
 
+template <typename Src>
+static hipError_t launch_convert_t(const ExpDesc *exps, int64_t n_exp, const Src *props,
+                                   ImgGauss *out, ImgGauss64 *out64, int4 *boxes,
+                                   uint32_t *status, hipStream_t stream) {
+    if (n_exp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(convert_kernel<Src>, dim3((unsigned)n_exp), dim3(128), 0, stream,
+                       exps, props, out, out64, boxes, status);
+    return hipGetLastError();
+}
+
 hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const fpho_source *props,
                           ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
                           hipStream_t stream) {
-    if (n_exp <= 0) return hipSuccess;
-    hipLaunchKernelGGL(convert_kernel, dim3((unsigned)n_exp), dim3(128), 0, stream,
-                       exps, props, out, out64, boxes, status);
-    return hipGetLastError();
+    return launch_convert_t(exps, n_exp, props, out, out64, boxes, status, stream);
+}
+
+hipError_t launch_convert(const ExpDesc *exps, int64_t n_exp, const Source64 *props,
+                          ImgGauss *out, ImgGauss64 *out64, int4 *boxes, uint32_t *status,
+                          hipStream_t stream) {
+    return launch_convert_t(exps, n_exp, props, out, out64, boxes, status, stream);
 }
 
 hipError_t launch_response(const double *dchi2, int32_t n_bands, int32_t n_src,
@@ -904,12 +938,12 @@ hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart
 }
 
 hipError_t launch_theta(const SrcDesc *srcs, int64_t n_srcs, const double *theta,
-                        const SplineView &sp, fpho_source *props, uint32_t *status,
-                        hipStream_t stream) {
+                        const SplineView &sp, fpho_source *props, Source64 *props64,
+                        uint32_t *status, hipStream_t stream) {
     if (n_srcs <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n_srcs + 63) / 64);
     hipLaunchKernelGGL(theta_kernel, dim3(blocks), dim3(64), 0, stream, srcs, (int32_t)n_srcs,
-                       theta, sp, props, status);
+                       theta, sp, props, props64, status);
     return hipGetLastError();
 }
 

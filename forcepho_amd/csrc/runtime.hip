@@ -166,6 +166,7 @@ struct Plan {
     DevVec<double> dchi2_tmp;       // single patch: dchi2 before the compat Response layout
     DevVec<SrcDesc> srcs;
     DevVec<PatchOut> pouts;
+    DevVec<Source64> src64;         // batch: FP64 source records of the theta path
     int64_t n_exp = 0, n_tiles = 0, n_outs = 0, n_gauss = 0, n_srcs = 0, n_patches = 0;
     int64_t n_pairs = 0, npix = 0;
     int32_t max_ng = 0;  // most image Gaussians of one exposure
@@ -484,6 +485,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     } else {
         plan.dchi2_tmp.release();
     }
+    HIP_TRY(plan.src64.alloc(std::max<int64_t>((int64_t)hp.srcs.size(), 1)));
     plan.n_exp = (int64_t)hp.exps.size();
     plan.n_tiles = (int64_t)hp.tiles.size();
     plan.n_outs = (int64_t)hp.outs.size();
@@ -525,7 +527,8 @@ EvalArgs eval_args(Plan &plan) {
     return a;
 }
 
-int run_plan(Plan &plan, const fpho_source *props, double *chi2, double *dchi2,
+template <typename Src>
+int run_plan(Plan &plan, const Src *props, double *chi2, double *dchi2,
              hipStream_t stream) {
     if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
     HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
@@ -1027,9 +1030,21 @@ int fpho_batch_theta_to_proposals_async(fpho_batch *b, const fpho_spline *s,
     if (!b || !s || !theta || !props) return fail(FPHO_E_ARG, "null argument");
     const int rc = ensure_batch_plan(b, (hipStream_t)stream);
     if (rc) return rc;
-    HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props,
+    HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props, nullptr,
                          b->plan.status.ptr, (hipStream_t)stream));
     return FPHO_OK;
+}
+
+int fpho_batch_evaluate_theta_async(fpho_batch *b, const fpho_spline *s, const double *theta,
+                                    fpho_source *props, double *chi2, double *dchi2,
+                                    void *stream) {
+    if (!b || !s || !theta || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
+    const int rc = ensure_batch_plan(b, (hipStream_t)stream);
+    if (rc) return rc;
+    HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props,
+                         b->plan.src64.ptr, b->plan.status.ptr, (hipStream_t)stream));
+    return run_plan(b->plan, (const Source64 *)b->plan.src64.ptr, chi2, dchi2,
+                    (hipStream_t)stream);
 }
 
 int fpho_batch_stack_grad_async(fpho_batch *b, const double *chi2, const double *dchi2,

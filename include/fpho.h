@@ -276,6 +276,19 @@ int fpho_batch_theta_to_proposals_async(fpho_batch *b, const fpho_spline *s,
                                         const double *theta_dev,
                                         fpho_source *proposals_dev,
                                         void *stream);
+/* theta -> chi2 and dchi2 of every patch in one stream-ordered call: the
+ * source records are built from theta as fpho_batch_theta_to_proposals_async
+ * does, but the conversion stage reads them in FP64 (the FP32 record rounds
+ * a source's ra/dec offset to ~3e-6 pixel, which moves the position gradient
+ * of a bright source by up to the stated tolerance); then the pixel pass and
+ * the reduction of fpho_batch_evaluate_async.  proposals_dev, if not NULL,
+ * receives the FP32 records too.  Replaces forcepho's per-evaluation
+ * Scene.get_proposal + EvaluateProposal round trip
+ * (forcepho/sources.py:80-97,607-694; compute_gaussians_kernel.cu:212-214)
+ * for callers that hold the parameter vector on the device. */
+int fpho_batch_evaluate_theta_async(fpho_batch *b, const fpho_spline *s,
+                                    const double *theta_dev, fpho_source *proposals_dev,
+                                    double *chi2_dev, double *dchi2_dev, void *stream);
 /* lnlike[i] = -1/2 sum_b chi2 of patch i, grad_theta = -1/2 * stacked dchi2
  * (forcepho/model.py:331-332, 370-387). */
 int fpho_batch_stack_grad_async(fpho_batch *b, const double *chi2_dev,

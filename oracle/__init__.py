@@ -442,28 +442,31 @@ def source_records64(sources, n_bands):
     return rec
 
 
-def evaluate64(patch, sources, cutoff=True):
+def evaluate64(patch, sources, cutoff=True, grad=True):
     """The C restatement fed FP64 source records (``sources_from_theta``
-    dicts): chi2 [n_bands], dchi2 [n_bands, n_sources, 7]."""
+    dicts): chi2 [n_bands], dchi2 [n_bands, n_sources, 7] (None with
+    ``grad=False``, which skips the gradient work)."""
     a = _arrays(patch)
     d = _desc(patch, a)
     nb, ns = len(a["band_N"]), int(patch.n_sources)
     rec = np.ascontiguousarray(source_records64(sources, nb))
     chi2 = np.zeros(nb)
-    dchi2 = np.zeros((nb, ns, 7))
+    dchi2 = np.zeros((nb, ns, 7)) if grad else None
     rc = load().oracle_evaluate64(ctypes.byref(d), rec.ctypes.data_as(_dp), int(bool(cutoff)),
-                                  chi2.ctypes.data_as(_dp), dchi2.ctypes.data_as(_dp), None,
-                                  None)
+                                  chi2.ctypes.data_as(_dp),
+                                  dchi2.ctypes.data_as(_dp) if grad else None, None, None)
     if rc != 0:
         raise RuntimeError("oracle_evaluate64 failed")
     return chi2, dchi2
 
 
-def lnlike_theta(patch, theta, cutoff=True):
+def lnlike_theta(patch, theta, cutoff=True, grad=True):
     """lnlike = -chi2/2 and d lnlike / d theta, all FP64 (scipy splines, FP64
-    source records, the C restatement)."""
+    source records, the C restatement); with ``grad=False`` only lnlike."""
     nb = len(np.asarray(patch.band_N))
-    chi2, dchi2 = evaluate64(patch, sources_from_theta(theta, nb), cutoff=cutoff)
+    chi2, dchi2 = evaluate64(patch, sources_from_theta(theta, nb), cutoff=cutoff, grad=grad)
+    if not grad:
+        return -0.5 * chi2.sum()
     return -0.5 * chi2.sum(), -0.5 * stack_grad(dchi2)
 
 

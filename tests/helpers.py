@@ -60,3 +60,34 @@ def make_small_patches(config=40):
     for i, spec in enumerate(small_uneven_specs()):
         out.append(synth.make_patch(spec, config=config, patch_id=i, render=oracle_render))
     return out
+
+
+# ----------------------------------------------------------- oracle fixtures
+class Fixture(dict):
+    """Arrays of a committed oracle fixture (tests/golden/*.npz, written by
+    tests/golden/make_scale_fixtures.py) and the digest it stores inputs by."""
+
+    @staticmethod
+    def sha256(a):
+        import hashlib
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load_fixture(path):
+    with np.load(path, allow_pickle=False) as f:
+        return Fixture({k: f[k] for k in f.files})
+
+
+def fixture_patches(specs, fx, config):
+    """{index: (patch, scene, theta)} of the fixture's patches, rebuilt from
+    their seeds with data = FP64 oracle model of the truth + seeded noise;
+    each patch's float32 data must hash to the fixture's digest."""
+    out = {}
+    for k, i in enumerate(fx["patch"]):
+        i = int(i)
+        patch, scene, theta = synth.make_patch(specs[i], config=config, patch_id=i,
+                                               render=oracle_render)
+        assert Fixture.sha256(patch.data) == fx["data_sha256"][k], \
+            f"patch {i}: rebuilt data differ from the fixture's; regenerate the fixtures"
+        out[i] = (patch, scene, theta)
+    return out

@@ -1,15 +1,19 @@
 """Parity of the HIP path against the FP64 oracle (and the compiled
 reference CPU kernel where it is built) on configurations 1 and 2."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle
 from forcepho_amd import synth
 from forcepho_amd.synth import PatchSpec
-from helpers import (CHI2_RTOL, chi2_relerr, grad_mismatch, make_config_patch,
-                     oracle_render, proposal_for)
+from helpers import (CHI2_RTOL, Fixture, chi2_relerr, grad_mismatch, load_fixture,
+                     make_config_patch, oracle_render, proposal_for)
 
 pytestmark = pytest.mark.gpu
+
+FIXTURE_C2 = os.path.join(os.path.dirname(__file__), "golden", "config2_proposals.npz")
 
 
 @pytest.fixture(scope="module")
@@ -68,12 +72,36 @@ def test_config1_against_reference_kernel(c1):
     assert not grad_mismatch(dchi2, d_r).any()
 
 
-def test_config2_proposals(c2):
+def test_config2_truth(c2):
     patch, scene, theta = c2
-    props = synth.make_proposals(theta, 8, 3, config=2)
     _check(patch, proposal_for(scene, theta), "c2 truth")
-    for k, t in enumerate(props):
-        _check(patch, proposal_for(scene, t), f"c2 proposal {k}")
+
+
+def test_config2_100_proposals_against_oracle_fixtures(c2):
+    """The 100 seeded proposals of configuration 2 against FP64 oracle chi2
+    and gradients committed in tests/golden/config2_proposals.npz (written
+    by tests/golden/make_scale_fixtures.py on the same seeded patch)."""
+    patch, scene, theta = c2
+    fx = load_fixture(FIXTURE_C2)
+    assert Fixture.sha256(patch.data) == fx["data_sha256"], \
+        "configuration 2's data differ from the fixture's; regenerate the fixtures"
+    props = [proposal_for(scene, t) for t in fx["theta"]]
+    assert Fixture.sha256(np.concatenate(props)) == fx["proposals_sha256"]
+    proposer = patch.get_proposer()
+    worst_c, worst_g = 0.0, 0.0
+    for k, prop in enumerate(props):
+        chi2, dchi2 = proposer.evaluate_proposal(prop, patch=patch)[:2]
+        c_ref, d_ref = fx["chi2"][k], fx["dchi2"][k]
+        rel = chi2_relerr(chi2, c_ref)
+        bad = grad_mismatch(dchi2, d_ref)
+        scale = np.abs(d_ref).reshape(len(d_ref), -1).max(axis=1)[:, None, None]
+        worst_c = max(worst_c, rel.max())
+        worst_g = max(worst_g, float(np.max(np.abs(dchi2 - d_ref) / scale)))
+        assert rel.max() <= CHI2_RTOL, (k, rel)
+        assert not bad.any(), (k, np.argwhere(bad)[:5])
+    print(f"c2, 100 proposals: chi2 rel err <= {worst_c:.2e}; gradient err <= "
+          f"{worst_g:.2e} of the band max; all 112000 components within tolerance",
+          flush=True)
 
 
 def test_determinism(c2):
