@@ -38,8 +38,9 @@ class DevicePatchMixin:
       big or fixed sources: load it, evaluate (the device writes data -
       model into the residual buffer) and swap data and residual, so the
       device data lose that block; finally load the active scene;
-    * ``design_matrix`` -- per active source, load the source alone with
-      unit fluxes and read its model back (the model is linear in flux);
+    * ``design_matrix`` -- per active source, load the source alone, read
+      its model back and scale it to unit flux (the model is linear in
+      flux);
     * ``render`` -- load a catalog and return minus the residual.
     """
 
@@ -102,7 +103,12 @@ class DevicePatchMixin:
         185-238): per band, X [n_active, n_pix_band] whose row i is source
         i's model at unit flux in that band, and the target data y (the
         device data, fixed sources already subtracted), so that
-        y_band ~ fluxes_band . X_band.  Returns (Xes, ys)."""
+        y_band ~ fluxes_band . X_band.  Returns (Xes, ys).
+
+        The model comes back as y - residual, whose float32 rounding is
+        absolute (a few ulp of the data), so each source is rendered at its
+        own fluxes and its rows are divided by them; a zero flux is rendered
+        at 1 instead."""
         y = self.retrieve_array("data")
         cuts = np.cumsum(self.band_N_pix)[:-1]
         ys = np.split(y, cuts)
@@ -111,14 +117,15 @@ class DevicePatchMixin:
         active = np.atleast_1d(active)
         Xes = [np.zeros((len(active), n)) for n in self.band_N_pix]
         for i in range(len(active)):
-            unit = active[i:i + 1].copy()       # a row of its own, not a view
-            for b in self.bandlist:
-                unit[b] = 1.0
-            scene = self._load_scene(self.set_scene(unit))
+            row = active[i:i + 1].copy()       # a row of its own, not a view
+            norm = np.array([float(row[b][0]) or 1.0 for b in self.bandlist])
+            for b, f in zip(self.bandlist, norm):
+                row[b] = f
+            scene = self._load_scene(self.set_scene(row))
             residual = proposer.evaluate_proposal(scene.get_proposal(), patch=self,
                                                   unpack=False)[2]
             for j, model_b in enumerate(np.split(y - residual, cuts)):
-                Xes[j][i] = model_b
+                Xes[j][i] = model_b / norm[j]
         return Xes, ys
 
     def render(self, catalog):
