@@ -27,7 +27,8 @@ EXPORTED = [
     "fpho_patch_read", "fpho_patch_info", "fpho_patch_set_launch",
     "fpho_patch_set_precision", "fpho_patch_set_residual_writes", "fpho_patch_status",
     "fpho_patch_free",
-    "fpho_evaluate", "fpho_evaluate_async", "fpho_evaluate_response",
+    "fpho_evaluate", "fpho_evaluate_async", "fpho_evaluate_async_noalloc",
+    "fpho_evaluate_response",
     "fpho_evaluate_response_async",
     "fpho_batch_create", "fpho_batch_layout", "fpho_batch_info",
     "fpho_batch_set_launch", "fpho_batch_set_precision",
@@ -104,6 +105,7 @@ def _declare(lib):
         "fpho_patch_free": ([P], c_int),
         "fpho_evaluate": ([P, P, c_double_p, c_float_p], c_int),
         "fpho_evaluate_async": ([P, P, P, P, P], c_int),
+        "fpho_evaluate_async_noalloc": ([P, P, P, P, P], c_int),
         "fpho_evaluate_response": ([P, P, c_double_p, P], c_int),
         "fpho_evaluate_response_async": ([P, P, P, P, P], c_int),
         "fpho_batch_create": ([ctypes.POINTER(P), c_int, ctypes.POINTER(P)], c_int),

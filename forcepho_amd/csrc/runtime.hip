@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -168,7 +169,7 @@ struct Plan {
     DevVec<PatchOut> pouts;
     DevVec<Source64> src64;         // batch: FP64 source records of the theta path
     int64_t n_exp = 0, n_tiles = 0, n_outs = 0, n_gauss = 0, n_srcs = 0, n_patches = 0;
-    int64_t n_pairs = 0, npix = 0;
+    int64_t n_pairs = 0, npix = 0, n_part = 0;
     int32_t max_ng = 0;  // most image Gaussians of one exposure
     std::vector<int64_t> src_off, chi2_off, dchi2_off, theta_off;  // per patch, +1
     fpho_launch launch;
@@ -199,6 +200,7 @@ struct fpho_patch {
     DevVec<float> data, ierr, xpix, ypix, residual;
     DevVec<float> rad2, D, crpix, crval, CW;
     DevVec<fpho_psf_gauss> psf;
+    DevVec<uint32_t> status;      // FPHO_STATUS_* bits of its evaluations
     bool write_residual = false;  // evaluations write residual = data - model
     uint64_t generation = 1;  // bumps on metadata change, a data/residual swap or
                               // a residual-write change
@@ -434,6 +436,16 @@ void plan_patch(const fpho_patch *p, int K, int64_t src_base, int64_t chi2_base,
     hp.pouts.push_back(po);
 }
 
+// Tile order of the queue: most expensive first (longest-processing-time
+// first); ties keep canonical order, so tiles of one exposure stay adjacent.
+std::vector<int32_t> cost_order(const HostPlan &hp) {
+    std::vector<int32_t> queue(hp.tiles.size());
+    std::iota(queue.begin(), queue.end(), 0);
+    std::stable_sort(queue.begin(), queue.end(),
+                     [&](int32_t a, int32_t b) { return hp.cost[a] > hp.cost[b]; });
+    return queue;
+}
+
 // On failure the plan stays invalid (its buffers may be partly replaced).
 int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &launch,
                Plan &plan) {
@@ -458,10 +470,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
         return fail(FPHO_E_LIMIT, "too many tiles (%zu)", hp.tiles.size());
     // Queue order: most expensive tiles first (longest-processing-time first);
     // ties keep canonical order, so tiles of one exposure stay adjacent.
-    std::vector<int32_t> queue(hp.tiles.size());
-    std::iota(queue.begin(), queue.end(), 0);
-    std::stable_sort(queue.begin(), queue.end(),
-                     [&](int32_t a, int32_t b) { return hp.cost[a] > hp.cost[b]; });
+    const std::vector<int32_t> queue = cost_order(hp);
 
     HIP_TRY(plan.exps.upload(hp.exps.data(), (int64_t)hp.exps.size()));
     HIP_TRY(plan.tiles.upload(hp.tiles.data(), (int64_t)hp.tiles.size()));
@@ -490,6 +499,7 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     plan.n_tiles = (int64_t)hp.tiles.size();
     plan.n_outs = (int64_t)hp.outs.size();
     plan.n_gauss = hp.n_gauss;
+    plan.n_part = hp.n_part;
     plan.max_ng = hp.max_ng;
     plan.n_srcs = (int64_t)hp.srcs.size();
     plan.n_patches = (int64_t)patches.size();
@@ -500,48 +510,146 @@ int build_plan(const std::vector<fpho_patch *> &patches, const fpho_launch &laun
     return FPHO_OK;
 }
 
-EvalArgs eval_args(Plan &plan) {
+// Device pointers and counts of a plan, wherever its buffers live (the
+// plan's own allocations, or the zero-allocation scratch area).
+struct PlanPtrs {
+    const ExpDesc *exps = nullptr;
+    const TileDesc *tiles = nullptr;
+    const int32_t *queue = nullptr;
+    const int4 *slot_boxes = nullptr;
+    const OutDesc *outs = nullptr;
+    ImgGauss *gauss = nullptr;
+    ImgGauss64 *gauss64 = nullptr;
+    int4 *gboxes = nullptr;
+    float *gpart = nullptr;
+    double *cpart = nullptr;
+    int32_t *counter = nullptr;
+    uint32_t *status = nullptr;
+    int64_t n_exp = 0, n_tiles = 0, n_outs = 0;
+    int32_t max_ng = 0;
+    fpho_launch launch;
+    int precision = kPrecMixed;
+    bool timing = false;
+    hipEvent_t *ev = nullptr;
+};
+
+PlanPtrs plan_ptrs(Plan &plan, uint32_t *status) {
+    PlanPtrs q;
+    q.exps = plan.exps.ptr;
+    q.tiles = plan.tiles.ptr;
+    q.queue = plan.queue.ptr;
+    q.slot_boxes = plan.slot_boxes.ptr;
+    q.outs = plan.outs.ptr;
+    q.gauss = plan.gauss.ptr;
+    q.gauss64 = plan.gauss64.ptr;
+    q.gboxes = plan.gboxes.ptr;
+    q.gpart = plan.gpart.ptr;
+    q.cpart = plan.cpart.ptr;
+    q.counter = plan.counter.ptr;
+    q.status = status;
+    q.n_exp = plan.n_exp;
+    q.n_tiles = plan.n_tiles;
+    q.n_outs = plan.n_outs;
+    q.max_ng = plan.max_ng;
+    q.launch = plan.launch;
+    q.precision = plan.precision;
+    q.timing = plan.timing;
+    q.ev = plan.ev;
+    return q;
+}
+
+EvalArgs eval_args(const PlanPtrs &q) {
     EvalArgs a;
-    a.tiles = plan.tiles.ptr;
-    a.queue = plan.queue.ptr;
-    a.n_tiles = plan.n_tiles;
-    a.counter = plan.counter.ptr;
-    a.exps = plan.exps.ptr;
-    a.gauss = plan.gauss.ptr;
-    a.gauss64 = plan.gauss64.ptr;
-    a.gboxes = plan.gboxes.ptr;
-    a.slot_boxes = plan.slot_boxes.ptr;
-    a.precision = plan.precision;
-    a.gpart = plan.gpart.ptr;
-    a.cpart = plan.cpart.ptr;
-    a.pix_per_lane = plan.launch.pix_per_lane;
-    a.cull = plan.launch.cull;
-    a.mask_cap = std::min(kMaskCapMax, (plan.max_ng + 7) & ~7);
-    a.act_words = std::min(kActWordsMax, (plan.max_ng + 63) / 64);
-    a.waves_per_group = plan.launch.waves_per_group;
-    a.persistent = plan.launch.persistent;
-    const int64_t groups_needed = (plan.n_tiles + a.waves_per_group - 1) / a.waves_per_group;
+    a.tiles = q.tiles;
+    a.queue = q.queue;
+    a.n_tiles = q.n_tiles;
+    a.counter = q.counter;
+    a.exps = q.exps;
+    a.gauss = q.gauss;
+    a.gauss64 = q.gauss64;
+    a.gboxes = q.gboxes;
+    a.slot_boxes = q.slot_boxes;
+    a.precision = q.precision;
+    a.gpart = q.gpart;
+    a.cpart = q.cpart;
+    a.pix_per_lane = q.launch.pix_per_lane;
+    a.cull = q.launch.cull;
+    a.mask_cap = std::min(kMaskCapMax, (q.max_ng + 7) & ~7);
+    a.act_words = std::min(kActWordsMax, (q.max_ng + 63) / 64);
+    a.waves_per_group = q.launch.waves_per_group;
+    a.persistent = q.launch.persistent;
+    const int64_t groups_needed = (q.n_tiles + a.waves_per_group - 1) / a.waves_per_group;
     a.grid = std::min<int64_t>(groups_needed,
-                               (int64_t)plan.launch.groups_per_cu * cu_count_cached());
+                               (int64_t)q.launch.groups_per_cu * cu_count_cached());
     if (a.grid < 1) a.grid = 1;
     return a;
 }
 
+EvalArgs eval_args(Plan &plan) { return eval_args(plan_ptrs(plan, plan.status.ptr)); }
+
+// convert -> pixel pass -> reduce on `stream`
 template <typename Src>
-int run_plan(Plan &plan, const Src *props, double *chi2, double *dchi2,
+int run_ptrs(const PlanPtrs &q, const Src *props, double *chi2, double *dchi2,
              hipStream_t stream) {
-    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[0], stream));
-    HIP_TRY(launch_convert(plan.exps.ptr, plan.n_exp, props, plan.gauss.ptr, plan.gauss64.ptr,
-                           plan.gboxes.ptr, plan.status.ptr, stream));
-    const EvalArgs a = eval_args(plan);
-    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[1], stream));
+    if (q.timing) HIP_TRY(hipEventRecord(q.ev[0], stream));
+    HIP_TRY(launch_convert(q.exps, q.n_exp, props, q.gauss, q.gauss64, q.gboxes, q.status,
+                           stream));
+    const EvalArgs a = eval_args(q);
+    if (q.timing) HIP_TRY(hipEventRecord(q.ev[1], stream));
     HIP_TRY(launch_eval(a, stream));
-    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[2], stream));
-    HIP_TRY(launch_reduce(plan.outs.ptr, plan.n_outs, plan.gpart.ptr, plan.cpart.ptr, chi2,
-                          dchi2, plan.precision != kPrecFP32, stream));
-    if (plan.timing) HIP_TRY(hipEventRecord(plan.ev[3], stream));
+    if (q.timing) HIP_TRY(hipEventRecord(q.ev[2], stream));
+    HIP_TRY(launch_reduce(q.outs, q.n_outs, q.gpart, q.cpart, chi2, dchi2,
+                          q.precision != kPrecFP32, stream));
+    if (q.timing) HIP_TRY(hipEventRecord(q.ev[3], stream));
     return FPHO_OK;
 }
+
+// A single patch's plan reports into the patch's status word, a batch's
+// into the plan's.
+template <typename Src>
+int run_plan(Plan &plan, uint32_t *status, const Src *props, double *chi2, double *dchi2,
+             hipStream_t stream) {
+    return run_ptrs(plan_ptrs(plan, status), props, chi2, dchi2, stream);
+}
+
+}  // namespace
+
+// ------------------------------------------- zero-allocation scratch area
+// The work buffers of fpho_evaluate_async_noalloc -- image Gaussians, their
+// FP64 copies and boxes, the tile partials and the work-queue counter --
+// live in this fixed array.  The HIP runtime places one copy on every GPU
+// the library is loaded on and keeps it until the process ends.  In host
+// code the array's name only stands in for the symbol: its address on a
+// device comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address is ever passed to kernels.
+constexpr int64_t kScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kScratchDevices] = {};
+
+// Address of the scratch array on `device`, which must be current; looked up
+// on the first call for that device and remembered.
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= kScratchDevices)
+        return fail(FPHO_E_LIMIT, "device %d outside 0..%d", device, kScratchDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(FPHO_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s",
+                        device, hipGetErrorString(e));
+        g_scratch_dev[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_dev[device];
+    return FPHO_OK;
+}
+
+int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 
 // True when `stream` is capturing a graph (plans are not rebuilt then: a
 // rebuild allocates and copies synchronously).
@@ -560,6 +668,7 @@ int ensure_patch_plan(fpho_patch *p, hipStream_t stream) {
     std::vector<fpho_patch *> one{p};
     const int rc = build_plan(one, p->launch, p->plan);
     if (rc) return rc;
+    HIP_TRY(hipMemset(p->status.ptr, 0, sizeof(uint32_t)));
     p->plan.precision = p->precision;
     p->plan_generation = p->generation;
     return FPHO_OK;
@@ -587,12 +696,13 @@ int ensure_batch_plan(fpho_batch *b, hipStream_t stream) {
     return FPHO_OK;
 }
 
-int read_status(Plan &plan, uint32_t *status, int clear) {
+// Wait for the device, then read (and with `clear`, reset) a status word.
+int read_status(uint32_t *word_dev, uint32_t *status, int clear) {
     *status = 0;
-    if (!plan.valid || !plan.status.ptr) return FPHO_OK;
+    if (!word_dev) return FPHO_OK;
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(status, plan.status.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (clear) HIP_TRY(hipMemset(plan.status.ptr, 0, sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(status, word_dev, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (clear) HIP_TRY(hipMemset(word_dev, 0, sizeof(uint32_t)));
     return FPHO_OK;
 }
 
@@ -631,7 +741,8 @@ int evaluate_sync(fpho_patch *p, const fpho_source *prop) {
     std::memcpy(p->prop_host.ptr, prop, sizeof(fpho_source) * p->n_sources);
     HIP_TRY(hipMemcpyAsync(p->prop_dev.ptr, p->prop_host.ptr,
                            sizeof(fpho_source) * p->n_sources, hipMemcpyHostToDevice, nullptr));
-    rc = run_plan(p->plan, p->prop_dev.ptr, p->out_dev.ptr, p->out_dev.ptr + p->n_bands,
+    rc = run_plan(p->plan, p->status.ptr, p->prop_dev.ptr, p->out_dev.ptr,
+                  p->out_dev.ptr + p->n_bands,
                   nullptr);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(p->out_host.ptr, p->out_dev.ptr, sizeof(double) * nout,
@@ -721,6 +832,11 @@ int fpho_patch_create(const fpho_patch_desc *d, int return_residual, fpho_patch 
             delete p;
             return fail(FPHO_E_HIP, "residual allocation failed: %s", hipGetErrorString(e));
         }
+    }
+    if ((e = p->status.alloc(1)) != hipSuccess ||
+        (e = hipMemset(p->status.ptr, 0, sizeof(uint32_t))) != hipSuccess) {
+        delete p;
+        return fail(FPHO_E_HIP, "status word allocation failed: %s", hipGetErrorString(e));
     }
     rc = upload_meta(p, d);
     if (rc) {
@@ -819,7 +935,7 @@ int fpho_patch_set_residual_writes(fpho_patch *p, int on) {
 
 int fpho_patch_status(fpho_patch *p, uint32_t *status, int clear) {
     if (!p || !status) return fail(FPHO_E_ARG, "null argument");
-    return read_status(p->plan, status, clear);
+    return read_status(p->status.ptr, status, clear);
 }
 
 int fpho_patch_free(fpho_patch *p) {
@@ -844,7 +960,55 @@ int fpho_evaluate_async(fpho_patch *p, const fpho_source *prop_dev, double *chi2
     if (!p || !prop_dev || !chi2_dev || !dchi2_dev) return fail(FPHO_E_ARG, "null argument");
     int rc = ensure_patch_plan(p, (hipStream_t)stream);
     if (rc) return rc;
-    return run_plan(p->plan, prop_dev, chi2_dev, dchi2_dev, (hipStream_t)stream);
+    return run_plan(p->plan, p->status.ptr, prop_dev, chi2_dev, dchi2_dev,
+                    (hipStream_t)stream);
+}
+
+int fpho_evaluate_async_noalloc(fpho_patch *p, const fpho_source *prop_dev, double *chi2_dev,
+                                double *dchi2_dev, void *stream) {
+    if (!p || !prop_dev || !chi2_dev || !dchi2_dev) return fail(FPHO_E_ARG, "null argument");
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (device != p->device)
+        return fail(FPHO_E_STATE, "the patch lives on device %d, device %d is current",
+                    p->device, device);
+    Plan &plan = p->plan;
+    if (!plan.valid || p->plan_generation != p->generation)
+        return fail(FPHO_E_STATE,
+                    "the patch changed since its descriptors were built; evaluate it once "
+                    "with fpho_evaluate_async (which allocates) before evaluating it here");
+    // work buffers in the scratch array, each 256-byte aligned
+    int64_t off = 0;
+    auto place = [&](int64_t bytes) {
+        const int64_t at = off;
+        off = align256(off + bytes);
+        return at;
+    };
+    const int64_t o_gauss = place((int64_t)sizeof(ImgGauss) * plan.n_gauss);
+    const int64_t o_gauss64 = place((int64_t)sizeof(ImgGauss64) * plan.n_gauss);
+    const int64_t o_gboxes = place((int64_t)sizeof(int4) * plan.n_gauss);
+    const int64_t o_gpart = place((int64_t)sizeof(float) * std::max<int64_t>(plan.n_part, 1));
+    const int64_t o_cpart = place((int64_t)sizeof(double) * std::max<int64_t>(plan.n_tiles, 1));
+    const int64_t o_counter = place((int64_t)sizeof(int32_t));
+    if (off > kScratchBytes)
+        return fail(FPHO_E_LIMIT,
+                    "the patch needs %lld bytes of work buffers (%lld image Gaussians, %lld "
+                    "tiles), more than the %lld-byte scratch area of the zero-allocation "
+                    "entry point; use fpho_evaluate_async",
+                    (long long)off, (long long)plan.n_gauss, (long long)plan.n_tiles,
+                    (long long)kScratchBytes);
+    unsigned char *base = nullptr;
+    const int rc = scratch_address(device, &base);
+    if (rc) return rc;
+    PlanPtrs q = plan_ptrs(plan, p->status.ptr);
+    q.gauss = reinterpret_cast<ImgGauss *>(base + o_gauss);
+    q.gauss64 = reinterpret_cast<ImgGauss64 *>(base + o_gauss64);
+    q.gboxes = reinterpret_cast<int4 *>(base + o_gboxes);
+    q.gpart = reinterpret_cast<float *>(base + o_gpart);
+    q.cpart = reinterpret_cast<double *>(base + o_cpart);
+    q.counter = reinterpret_cast<int32_t *>(base + o_counter);
+    q.timing = false;
+    return run_ptrs(q, prop_dev, chi2_dev, dchi2_dev, (hipStream_t)stream);
 }
 
 int fpho_evaluate_response(fpho_patch *p, const fpho_source *prop, double *chi2,
@@ -873,7 +1037,8 @@ int fpho_evaluate_response_async(fpho_patch *p, const fpho_source *prop_dev, dou
                     p->n_sources, FPHO_MAXSOURCES);
     int rc = ensure_patch_plan(p, (hipStream_t)stream);
     if (rc) return rc;
-    rc = run_plan(p->plan, prop_dev, chi2_dev, p->plan.dchi2_tmp.ptr, (hipStream_t)stream);
+    rc = run_plan(p->plan, p->status.ptr, prop_dev, chi2_dev, p->plan.dchi2_tmp.ptr,
+                  (hipStream_t)stream);
     if (rc) return rc;
     HIP_TRY(launch_response(p->plan.dchi2_tmp.ptr, p->n_bands, p->n_sources, resp_dev,
                             (hipStream_t)stream));
@@ -948,7 +1113,7 @@ int fpho_batch_evaluate_async(fpho_batch *b, const fpho_source *props, double *c
     if (!b || !props || !chi2 || !dchi2) return fail(FPHO_E_ARG, "null argument");
     const int rc = ensure_batch_plan(b, (hipStream_t)stream);
     if (rc) return rc;
-    return run_plan(b->plan, props, chi2, dchi2, (hipStream_t)stream);
+    return run_plan(b->plan, b->plan.status.ptr, props, chi2, dchi2, (hipStream_t)stream);
 }
 
 int fpho_batch_set_timing(fpho_batch *b, int on) {
@@ -972,7 +1137,7 @@ int fpho_batch_last_timing(fpho_batch *b, float *ms3) {
 
 int fpho_batch_status(fpho_batch *b, uint32_t *status, int clear) {
     if (!b || !status) return fail(FPHO_E_ARG, "null argument");
-    return read_status(b->plan, status, clear);
+    return read_status(b->plan.valid ? b->plan.status.ptr : nullptr, status, clear);
 }
 
 int fpho_batch_free(fpho_batch *b) {
@@ -1043,7 +1208,8 @@ int fpho_batch_evaluate_theta_async(fpho_batch *b, const fpho_spline *s, const d
     if (rc) return rc;
     HIP_TRY(launch_theta(b->plan.srcs.ptr, b->plan.n_srcs, theta, s->view, props,
                          b->plan.src64.ptr, b->plan.status.ptr, (hipStream_t)stream));
-    return run_plan(b->plan, (const Source64 *)b->plan.src64.ptr, chi2, dchi2,
+    return run_plan(b->plan, b->plan.status.ptr, (const Source64 *)b->plan.src64.ptr, chi2,
+                    dchi2,
                     (hipStream_t)stream);
 }
 

@@ -204,6 +204,22 @@ int fpho_evaluate(fpho_patch *p, const fpho_source *proposal_host,
 int fpho_evaluate_async(fpho_patch *p, const fpho_source *proposal_dev,
                         double *chi2_dev, double *dchi2_dev, void *stream);
 
+/* Zero-allocation variant of fpho_evaluate_async.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during an
+ * evaluation (tight serving or sampling loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results bit for bit those of
+ * fpho_evaluate_async.  The image Gaussians, tile partials and queue counter
+ * live in a fixed 64 MiB scratch array that every GPU the library runs on
+ * holds until the process ends; a patch whose work buffers do not fit returns
+ * FPHO_E_LIMIT.  The tile descriptors are the patch's own: after any change to
+ * the patch (new metadata, a swap, a launch or residual-write change) call
+ * fpho_evaluate_async once, which rebuilds them; until then this returns
+ * FPHO_E_STATE.  All calls on one GPU share its scratch array, so calls that
+ * overlap on one GPU (on different streams or threads) race: order them on
+ * one stream.  Invalid proposal values set the patch's status word. */
+int fpho_evaluate_async_noalloc(fpho_patch *p, const fpho_source *proposal_dev,
+                                double *chi2_dev, double *dchi2_dev, void *stream);
+
 /* The reference's output layout, for callers written against
  * EvaluateProposal(patch, proposal, pchi2, pdchi2_dp)
  * (forcepho/src/compute_gaussians_kernel.cu:212-214, 304-310): chi2[n_bands]
