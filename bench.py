@@ -463,16 +463,16 @@ def main():
     chi2_err = None
     try:
         import oracle
-        from forcepho_amd.limits import source_struct_dtype
-        props = batch.proposals.cpu().numpy().view(source_struct_dtype)
         chi2_dev = batch.chi2_host()
         small = [int(i) for i in np.argsort([s.n_pairs() for s in specs])[:3]]
         errs = []
         for i in small:
             p, sc, th = patches[i]
             p.data = p.retrieve_array("data")
-            prop = props[batch.src_off[i]:batch.src_off[i + 1]]
-            c_ref, _ = oracle.evaluate(p, prop)
+            # the sweep converts FP64 source records made from theta: so does
+            # the oracle here (scipy splines, FP64 sources)
+            c_ref, _ = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i],
+                                                                      len(p.band_N)))
             errs.append(np.max(np.abs(chi2_dev[i] - c_ref) / np.abs(c_ref)))
             p.data = None
         chi2_err = {"max_rel_err": float(max(errs)), "patches": small,
@@ -588,7 +588,6 @@ def run_config5(args):
     from forcepho_amd import _lib, synth
     from forcepho_amd.autotune import load_or_default
     from forcepho_amd.batch import DeviceSplines, PatchBatch
-    from forcepho_amd.limits import source_struct_dtype
 
     _lib.init(0)
     torch.cuda.init()
@@ -613,15 +612,14 @@ def run_config5(args):
               for i, (p, sc, th) in enumerate(patches)]
     theta = torch.as_tensor(np.concatenate(thetas), dtype=torch.float64, device="cuda")
 
-    # FP64 oracle on the sample (proposals made on the device from theta)
-    batch.theta_to_proposals(theta, splines)
-    props = batch.proposals.cpu().numpy().view(source_struct_dtype)
+    # FP64 oracle on the sample, fed FP64 source records made from theta as
+    # the sweep's conversion stage is (scipy splines)
     ref = {}
     for i in sample:
         p = patches[i][0]
         p.data = p.retrieve_array("data")
         t0 = time.time()
-        ref[i] = oracle.evaluate(p, props[batch.src_off[i]:batch.src_off[i + 1]])
+        ref[i] = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i], len(p.band_N)))
         log(f"oracle patch {i}: {time.time() - t0:.1f}s")
 
     modes = {}
