@@ -55,6 +55,7 @@ class DevicePosterior:
         self.lower = torch.as_tensor(np.asarray(lower, dtype=np.float64), device=dev)
         self.upper = torch.as_tensor(np.asarray(upper, dtype=np.float64), device=dev)
         self.range = self.upper - self.lower
+        self.log_range = torch.log(self.range)
         idx, mask = segment_sum_index(batch.theta_off)
         self.seg_idx = torch.as_tensor(idx, device=dev)
         self.seg_mask = torch.as_tensor(mask, dtype=torch.float64, device=dev)
@@ -76,7 +77,12 @@ class DevicePosterior:
         q = self.lower + y * self.range
         jac = self.range * y * (1.0 - y)
         ll, gq = self.batch.lnlike_and_grad(q, self.splines)
-        lnp = ll + self.segment_sum(torch.log(jac))
+        # log |dq/dz| = log(range) + log(y (1 - y)), the latter written as
+        # -|z| - 2 log(1 + exp(-|z|)) so that it stays finite where y (1 - y)
+        # underflows (a parameter driven far against a bound)
+        a = torch.abs(z)
+        lnjac = self.log_range - a - 2.0 * torch.log1p(torch.exp(-a))
+        lnp = ll + self.segment_sum(lnjac)
         grad = gq * jac + (1.0 - 2.0 * y)
         return lnp, grad
 
