@@ -270,6 +270,52 @@ def time_hmc(patches, batch, splines, thetas, n_traj=3, n_steps=10, eps=1e-3):
             "acceptance": float(hmc.n_accept.mean().item()) / (n_traj + 1)}
 
 
+# ------------------------------------------------------------------- CPU leg
+# Everything in bench.py that reads oracle/ is in this leg: the reference's
+# own CPU kernel (compiled into oracle/_ref) timed as the baseline, and the
+# FP64 oracle as the checker of the sweep's results on a sample of patches.
+# Nothing the benchmark times runs through it; --skip-cpu skips it.
+
+def oracle_chi2_check(patches, specs, thetas, batch):
+    """Relative chi2 error of the last sweep on the three smallest patches
+    against the FP64 oracle fed the FP64 source records the sweep converts
+    (scipy splines), or None without the oracle."""
+    try:
+        import oracle
+    except ImportError:
+        return None
+    chi2_dev = batch.chi2_host()
+    small = [int(i) for i in np.argsort([s.n_pairs() for s in specs])[:3]]
+    errs = []
+    for i in small:
+        p = patches[i][0]
+        p.data = p.retrieve_array("data")
+        c_ref, _ = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i], len(p.band_N)))
+        errs.append(np.max(np.abs(chi2_dev[i] - c_ref) / np.abs(c_ref)))
+        p.data = None
+    return {"max_rel_err": float(max(errs)), "patches": small,
+            "n_pairs": [specs[i].n_pairs() for i in small],
+            "note": "the three smallest patches, FP64 oracle on the host; the largest, "
+                    "median and most-sourced patches are checked against committed oracle "
+                    "fixtures by the GPU tests"}
+
+
+def oracle_reference(patches, thetas, sample):
+    """FP64 oracle chi2 and dchi2 of the sample patches at their parameter
+    vectors (FP64 source records, as the sweep converts), {index: (chi2,
+    dchi2)}."""
+    import oracle
+    ref = {}
+    for i in sample:
+        p = patches[i][0]
+        p.data = p.retrieve_array("data")
+        t0 = time.time()
+        ref[i] = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i], len(p.band_N)))
+        p.data = None
+        log(f"oracle patch {i}: {time.time() - t0:.1f}s")
+    return ref
+
+
 def cpu_baseline(patches, specs, thetas, seconds):
     """The reference's own CPU kernel (compiled from its sources into
     oracle/_ref), one thread, on a seeded random subset of the patches whose
@@ -459,30 +505,6 @@ def main():
         f"algorithmic {flops_alg:.3e} FLOP per sweep")
     log(f"stages (ms): convert {stage[0]:.3f} pixel pass {stage[1]:.3f} reduce {stage[2]:.3f}")
 
-    # ---- chi2 error on a sample of the smallest patches vs the FP64 oracle
-    chi2_err = None
-    try:
-        import oracle
-        chi2_dev = batch.chi2_host()
-        small = [int(i) for i in np.argsort([s.n_pairs() for s in specs])[:3]]
-        errs = []
-        for i in small:
-            p, sc, th = patches[i]
-            p.data = p.retrieve_array("data")
-            # the sweep converts FP64 source records made from theta: so does
-            # the oracle here (scipy splines, FP64 sources)
-            c_ref, _ = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i],
-                                                                      len(p.band_N)))
-            errs.append(np.max(np.abs(chi2_dev[i] - c_ref) / np.abs(c_ref)))
-            p.data = None
-        chi2_err = {"max_rel_err": float(max(errs)), "patches": small,
-                    "n_pairs": [specs[i].n_pairs() for i in small],
-                    "note": "the three smallest patches, FP64 oracle on the host; the "
-                            "largest, median and most-sourced patches are checked "
-                            "against committed oracle fixtures by the GPU tests"}
-    except ImportError:
-        pass
-
     # ---- graph-captured HMC leapfrog over every patch (10 steps/trajectory)
     hmc_rate = None
     if not args.skip_hmc:
@@ -501,8 +523,10 @@ def main():
         plain = len(patches) / t_plain
         log(f"plain: {t_plain * 1e3:.1f} ms/sweep, {plain:.1f} evals/s")
 
-    cpu = cpu_all = None
+    cpu = cpu_all = chi2_err = None
     if not args.skip_cpu:
+        batch.lnlike_and_grad(theta, splines)   # the sweep the checker reads back
+        chi2_err = oracle_chi2_check(patches, specs, thetas, batch)
         c = cpu_baseline(patches, specs, thetas, args.cpu_seconds)
         if c is not None:
             t_sweep = pairs / c["pairs_per_s"]
@@ -584,7 +608,6 @@ def run_config5(args):
     graph-captured HMC trajectories over all patches, and the plain drop-in
     timed on a random sample of patches.  Prints one JSON line."""
     import torch
-    import oracle
     from forcepho_amd import _lib, synth
     from forcepho_amd.autotune import load_or_default
     from forcepho_amd.batch import DeviceSplines, PatchBatch
@@ -612,15 +635,8 @@ def run_config5(args):
               for i, (p, sc, th) in enumerate(patches)]
     theta = torch.as_tensor(np.concatenate(thetas), dtype=torch.float64, device="cuda")
 
-    # FP64 oracle on the sample, fed FP64 source records made from theta as
-    # the sweep's conversion stage is (scipy splines)
-    ref = {}
-    for i in sample:
-        p = patches[i][0]
-        p.data = p.retrieve_array("data")
-        t0 = time.time()
-        ref[i] = oracle.evaluate64(p, oracle.sources_from_theta(thetas[i], len(p.band_N)))
-        log(f"oracle patch {i}: {time.time() - t0:.1f}s")
+    # the CPU leg's checker: FP64 oracle results of the sample
+    ref = {} if args.skip_cpu else oracle_reference(patches, thetas, sample)
 
     modes = {}
     for mode in ("mixed", "fp64_reduce", "fp32"):
@@ -635,7 +651,7 @@ def run_config5(args):
         chi2 = batch.chi2_host()
         dchi2 = batch.dchi2_host()
         cerr, gbad, gsize, gerr = 0.0, 0, 0, 0.0
-        for i in sample:
+        for i in ref:
             c_ref, d_ref = ref[i]
             cerr = max(cerr, float(np.max(np.abs(chi2[i] - c_ref) / np.abs(c_ref))))
             d = dchi2[i].astype(np.float64)
