@@ -44,7 +44,8 @@ class PatchSpec:
 
     def __init__(self, n_sources, n_bands, n_exp_per_band, side, n_psf,
                  superpixel=(8, 8), warp_size=64, mask_frac=0.0, psf_per_exposure=False,
-                 psf_elliptical=False, psf_sigma=(0.5, 3.0), dither=0.0, pix_offset=0.0):
+                 psf_elliptical=False, psf_sigma=(0.5, 3.0), dither=0.0, pix_offset=0.0,
+                 psf_offset=0.0):
         self.n_sources = int(n_sources)
         self.n_bands = int(n_bands)
         self.n_exp_per_band = np.broadcast_to(np.asarray(n_exp_per_band, dtype=int),
@@ -57,6 +58,7 @@ class PatchSpec:
         self.psf_per_exposure = bool(psf_per_exposure)
         self.psf_elliptical = bool(psf_elliptical)
         self.psf_sigma = tuple(psf_sigma)
+        self.psf_offset = float(psf_offset)   # PSF component centres U[-o, o] pixels
         self.dither = float(dither)
         self.pix_offset = float(pix_offset)   # added to every pixel coordinate
         self.crowded = False
@@ -124,7 +126,7 @@ def config5_specs(n_patches=4096, crowded_frac=0.05):
     return specs
 
 
-def _psf_rows(rng, n_psf, sigma_range, elliptical):
+def _psf_rows(rng, n_psf, sigma_range, elliptical, offset=0.0):
     amps = rng.dirichlet(np.full(n_psf, 2.0))
     sig = rng.uniform(sigma_range[0], sigma_range[1], size=n_psf)
     rows = np.zeros((N_RADII, n_psf), dtype=psf_struct_dtype)
@@ -142,6 +144,10 @@ def _psf_rows(rng, n_psf, sigma_range, elliptical):
         rows["Cyy"][:, g] = C[1, 1]
         rows["Cxy"][:, g] = C[0, 1]
     rows["sersic_bin"] = np.arange(N_RADII)[:, None]
+    if offset:
+        # a component's centre offset is the same in every radius bin
+        rows["xcen"] = rng.uniform(-offset, offset, size=n_psf)[None, :]
+        rows["ycen"] = rng.uniform(-offset, offset, size=n_psf)[None, :]
     return rows.reshape(-1)
 
 
@@ -189,7 +195,7 @@ def make_patch(spec, config=1, patch_id=0, return_residual=False, render=None):
             wcses.append(AffineWCS(crpix, (ra0, dec0), cd))
             if spec.psf_per_exposure or b not in band_psf:
                 band_psf[b] = _psf_rows(rng, int(spec.n_psf[b]), spec.psf_sigma,
-                                        spec.psf_elliptical)
+                                        spec.psf_elliptical, spec.psf_offset)
             psfs.append(band_psf[b])
             bands.append(b)
             ie = np.float32(1.0 / sigmas[b])

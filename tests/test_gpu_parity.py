@@ -160,3 +160,36 @@ def test_large_pixel_coordinates(hip):
         _check(patch, proposal_for(scene, t), "offset 4000 proposal")
     finally:
         patch.free()
+
+
+def test_offset_psf_components(hip):
+    """PSF mixture components centred off the source (as real PSF fits are),
+    per-exposure flux calibrations G != 1, several bands, per-exposure
+    elliptical PSFs and dithers: the drop-in and
+    the batched theta path against the FP64 oracle."""
+    from forcepho_amd.batch import DeviceSplines, PatchBatch
+    import torch
+    spec = PatchSpec(n_sources=5, n_bands=3, n_exp_per_band=[2, 1, 2], side=64,
+                     n_psf=[3, 2, 4], psf_per_exposure=True, psf_elliptical=True,
+                     psf_offset=1.5, dither=0.5, mask_frac=0.02)
+    patch, scene, theta = synth.make_patch(spec, config=45, patch_id=0, render=oracle_render)
+    assert abs(patch.psfgauss["xcen"]).max() > 0.5
+    patch.G = np.array([0.9, 1.1, 1.0, 1.2, 0.95], dtype=np.float32)   # calibrations
+    patch.send_to_device()
+    batch = PatchBatch([patch])
+    splines = DeviceSplines()
+    try:
+        for k, t in enumerate(np.vstack([theta[None],
+                                         synth.make_proposals(theta, 3, 3, config=45)])):
+            _check(patch, proposal_for(scene, t), f"offset PSF proposal {k}")
+            tt = torch.as_tensor(t, dtype=torch.float64, device="cuda")
+            ll, g = batch.lnlike_and_grad(tt, splines)
+            ll_ref, g_ref = oracle.lnlike_theta(patch, t)
+            g = g.cpu().numpy()
+            err = np.abs(g - g_ref)
+            assert abs(ll.item() - ll_ref) <= CHI2_RTOL * abs(ll_ref)
+            assert np.all((err <= 1e-4 * np.abs(g_ref)) | (err <= 1e-6 * np.abs(g_ref).max()))
+    finally:
+        splines.free()
+        batch.free()
+        patch.free()

@@ -67,6 +67,60 @@ def test_against_reference_kernel_live(small_multiband):
         assert np.abs(d[b] - dr[b]).max() <= 2e-5 * np.abs(d[b]).max()
 
 
+def _offset_psf_patch():
+    from helpers import oracle_render
+    spec = synth.PatchSpec(n_sources=3, n_bands=2, n_exp_per_band=[1, 2], side=40,
+                           n_psf=[3, 2], psf_per_exposure=True, psf_elliptical=True,
+                           psf_offset=1.5, dither=0.5)
+    patch, scene, theta = synth.make_patch(spec, config=9, patch_id=0, render=oracle_render)
+    patch.G = np.array([0.9, 1.15, 1.05], dtype=np.float32)   # per-exposure calibrations
+    return patch, scene, theta
+
+
+def test_offset_psf_components_finite_differences():
+    """PSF mixtures whose components are not centred on the source (real
+    PSF fits have offsets; the synthetic workloads do not), and flux
+    calibrations G != 1: the oracle's
+    position (and every other) gradient equals FP64 central differences,
+    and its C and numpy restatements agree."""
+    patch, scene, theta = _offset_psf_patch()
+    assert np.abs(patch.psfgauss["xcen"]).max() > 0.5
+    ll, g = oracle.lnlike_theta(patch, theta, cutoff=False)
+    for i in range(len(theta)):
+        h = 1e-7 * max(abs(theta[i]), 1e-4)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (oracle.lnlike_theta(patch, tp, cutoff=False, grad=False) -
+              oracle.lnlike_theta(patch, tm, cutoff=False, grad=False)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * np.abs(g).max(), (i, fd, g[i])
+    srcs = oracle.sources_from_theta(theta, 2)
+    c2, d2, _ = oracle.evaluate_np(patch, srcs)
+    c, d = oracle.evaluate64(patch, srcs)
+    assert np.allclose(c2, c, rtol=1e-12)
+    assert np.abs(d2 - d).max() <= 1e-9 * np.abs(d).max()
+
+
+def test_offset_psf_components_against_reference_kernel():
+    """The compiled reference kernel on offset PSF components: the same
+    model (chi2 to 1e-6), and gradients within its FP32 accuracy.  The
+    position gradient of the brightest source here is off by 2.2e-4 of the
+    band's largest component in the reference; a float32 emulation of its
+    per-pair arithmetic reproduces that to 4e-6 of the band max, and the
+    oracle equals FP64 central differences (test above), so the difference
+    is the reference's FP32 arithmetic, not the offsets' semantics."""
+    if oracle.reference_module() is None:
+        pytest.skip("compiled reference kernel not built")
+    patch, scene, theta = _offset_psf_patch()
+    for t in np.vstack([theta[None], synth.make_proposals(theta, 2, 2, config=9)]):
+        prop = proposal_for(scene, t)
+        c, d = oracle.evaluate(patch, prop)
+        cr, dr = oracle.ReferenceKernel(patch).evaluate(prop)
+        assert np.all(np.abs(c - cr) <= 1e-6 * np.abs(c))
+        for b in range(len(c)):
+            assert np.abs(d[b] - dr[b]).max() <= 5e-4 * np.abs(d[b]).max()
+
+
 @pytest.mark.parametrize("cutoff,rel_step,tol", [(False, 1e-5, 1e-6), (True, 1e-6, 1e-6)])
 def test_gradient_central_differences(c1, cutoff, rel_step, tol):
     """d lnlike / d theta (FP64, through the amplitude splines) against central
