@@ -230,7 +230,8 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 //   MIXED = true : residual accumulated in FP64; a term whose FP32 value
 //                  exceeds kBrightSigma * sigma of the tile's quietest pixel
 //                  is re-evaluated in FP64 (a wave branch taken only near
-//                  bright cores; about 3% of slot evaluations on config 4).
+//                  bright cores; about 3% of slot evaluations on 512 uneven
+//                  patches of 32^2-256^2 pixel exposures).
 //   MIXED = false: FP32 residual, as forcepho/src/common_kernel.cc:57-83.
 //
 // Scaled variables (ImgGauss): d = pixel - centre, v' = s F d, E' = d.v' =

@@ -4,7 +4,7 @@ The packers read exposure metadata as FITS headers: the metadata store keeps
 them as header strings (forcepho/patches/storage.py:302-522, written with
 ``Header.tostring`` and read back with ``Header.fromstring``) and the FITS
 packer reads image extensions (forcepho/patches/pixel_patch.py:316-483).
-astropy is not part of this build, so this module implements the subset of
+This package does not depend on astropy, so this module implements the subset of
 the FITS standard (version 4.0) those paths use:
 
 * 80-character header cards: ``KEYWORD = value / comment`` with string

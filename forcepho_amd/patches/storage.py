@@ -2,7 +2,7 @@
 (forcepho/patches/storage.py).
 
 The reference keeps pixels and PSFs in HDF5 files and headers in a JSON
-file.  h5py is not part of this build, so the two array stores here are
+file.  This package does not depend on h5py, so the two array stores here are
 directory trees with the same group layout, one ``.npy`` file per dataset
 (read memory-mapped, never unpickled) and a JSON file per attribute set:
 

@@ -3,7 +3,7 @@
 The packers need, per exposure, the sky-to-pixel map at each source
 (``world_to_pixel_values``) and its local linear approximation
 (:func:`scale_at_sky`, forcepho/slow/stamp.py:200-256).  The reference
-takes these from astropy's WCS; astropy is not part of this build, so this
+takes these from astropy's WCS; this package does not depend on astropy, so this
 module implements the gnomonic (TAN) projection of the FITS WCS standard
 (Calabretta & Greisen 2002, A&A 395, 1077, eq. 54 with the rotation of
 eqs. 2 and 5) with a CD (or PC x CDELT) matrix, in the equivalent
