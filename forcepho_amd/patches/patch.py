@@ -277,10 +277,11 @@ class Patch(PatchBase):
     """
 
     def __init__(self, splinedata=None, return_residual=False,
-                 meta_dtype=np.float32, pix_dtype=np.float32):
+                 meta_dtype=np.float32, pix_dtype=np.float32, spline_smoothing=None):
         super().__init__(return_residual=return_residual, meta_dtype=meta_dtype,
                          pix_dtype=pix_dtype)
         self.splinedata = splinedata
+        self.spline_smoothing = spline_smoothing
         self.patch_reference_coordinates = np.zeros(2)
 
     # -- pixels
@@ -327,9 +328,12 @@ class Patch(PatchBase):
         self.scene = scene
 
     def set_scene(self, sourcecat, splinedata=None):
-        """A Scene of Galaxy sources from a catalog with this patch's bands."""
+        """A Scene of Galaxy sources from a catalog with this patch's bands
+        (amplitude splines of ``splinedata`` with this patch's
+        ``spline_smoothing``, forcepho/patches/patch.py:392-422)."""
         return Scene(catalog=sourcecat, filternames=self.bandlist,
-                     splinedata=splinedata or self.splinedata)
+                     splinedata=splinedata or self.splinedata,
+                     spline_smoothing=self.spline_smoothing)
 
     def zerocoords(self, scene, sky_zero=None):
         """Make the scene's source coordinates offsets from ``sky_zero``

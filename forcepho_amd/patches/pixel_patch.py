@@ -67,9 +67,10 @@ class PixelPatch(Patch):
     :meth:`find_exposures` and :meth:`find_pixels`."""
 
     def __init__(self, psfstore="", splinedata=None, return_residual=False,
-                 meta_dtype=np.float32, pix_dtype=np.float32, debug=0):
+                 meta_dtype=np.float32, pix_dtype=np.float32, debug=0, spline_smoothing=None):
         super().__init__(splinedata=splinedata or None, return_residual=return_residual,
-                         meta_dtype=meta_dtype, pix_dtype=pix_dtype)
+                         meta_dtype=meta_dtype, pix_dtype=pix_dtype,
+                         spline_smoothing=spline_smoothing)
         if isinstance(psfstore, str):
             psfstore = PSFStore(psfstore) if psfstore else None
         self.psfstore = psfstore
@@ -163,10 +164,11 @@ class StorePatch(PixelPatch):
 
     def __init__(self, pixelstore="", metastore="", psfstore="", splinedata=None,
                  return_residual=False, meta_dtype=np.float32, pix_dtype=np.float32,
-                 debug=0):
+                 debug=0, spline_smoothing=None):
         super().__init__(psfstore=psfstore, splinedata=splinedata,
                          return_residual=return_residual, meta_dtype=meta_dtype,
-                         pix_dtype=pix_dtype, debug=debug)
+                         pix_dtype=pix_dtype, debug=debug,
+                         spline_smoothing=spline_smoothing)
         self.metastore = metastore if isinstance(metastore, MetaStore) else MetaStore(metastore)
         self.pixelstore = pixelstore if isinstance(pixelstore, PixelStore) \
             else PixelStore(pixelstore)
@@ -217,10 +219,11 @@ class FITSPatch(PixelPatch):
 
     def __init__(self, fitsfiles=(), psfstore="", splinedata=None, return_residual=False,
                  sci_ext="SCI", unc_ext="ERR", meta_dtype=np.float32, pix_dtype=np.float32,
-                 debug=0):
+                 debug=0, spline_smoothing=None):
         super().__init__(psfstore=psfstore, splinedata=splinedata,
                          return_residual=return_residual, meta_dtype=meta_dtype,
-                         pix_dtype=pix_dtype, debug=debug)
+                         pix_dtype=pix_dtype, debug=debug,
+                         spline_smoothing=spline_smoothing)
         self.fitsfiles = list(fitsfiles)
         self.sci_ext = sci_ext
         self.unc_ext = unc_ext
@@ -283,10 +286,11 @@ class DirectPatch(PixelPatch):
 
     def __init__(self, pixel_dict=None, header_dict=None, psfstore="", splinedata=None,
                  return_residual=False, meta_dtype=np.float32, pix_dtype=np.float32,
-                 debug=0):
+                 debug=0, spline_smoothing=None):
         super().__init__(psfstore=psfstore, splinedata=splinedata,
                          return_residual=return_residual, meta_dtype=meta_dtype,
-                         pix_dtype=pix_dtype, debug=debug)
+                         pix_dtype=pix_dtype, debug=debug,
+                         spline_smoothing=spline_smoothing)
         self.pixel_dict = pixel_dict or {}
         self.header_dict = header_dict or {}
 

@@ -25,7 +25,8 @@ class Galaxy:
     npos = 2
     nshape = 4
 
-    def __init__(self, filters=("band",), filternames=None, splinedata=None, **kwargs):
+    def __init__(self, filters=("band",), filternames=None, splinedata=None,
+                 spline_smoothing=None, spline_type="rect", **kwargs):
         self.filternames = list(filternames if filternames is not None else filters)
         self.flux = np.zeros(len(self.filternames))
         self.ra = self.dec = 0.0
@@ -33,7 +34,8 @@ class Galaxy:
         self.sersic, self.rhalf = 2.0, 0.1
         self.fixed = False
         self.id = 0
-        self.splines = get_splines(splinedata or DEFAULT_SPLINEDATA)
+        self.splines = get_splines(splinedata or DEFAULT_SPLINEDATA,
+                                   spline_smoothing=spline_smoothing, spline_type=spline_type)
         self.radii = self.splines.radii
         self.proposal_struct = np.zeros(1, dtype=source_struct_dtype)
 
@@ -112,19 +114,22 @@ class Scene:
     """An ordered collection of sources and the flat parameter vector."""
 
     def __init__(self, sources=(), catalog=None, filternames=None, splinedata=None,
-                 **kwargs):
+                 spline_smoothing=None, spline_type="rect", **kwargs):
         self.sources = list(sources)
         if catalog is not None:
-            self.from_catalog(catalog, filternames=filternames, splinedata=splinedata)
+            self.from_catalog(catalog, filternames=filternames, splinedata=splinedata,
+                              spline_smoothing=spline_smoothing, spline_type=spline_type)
         for i, s in enumerate(self.sources):
             s.id = i
 
-    def from_catalog(self, catalog, filternames, splinedata=None):
+    def from_catalog(self, catalog, filternames, splinedata=None, spline_smoothing=None,
+                     spline_type="rect"):
         """Sources from a structured array with SHAPE_COLS and one flux column
         per filter name."""
         self.sources = []
         for row in catalog:
-            g = Galaxy(filternames=filternames, splinedata=splinedata)
+            g = Galaxy(filternames=filternames, splinedata=splinedata,
+                       spline_smoothing=spline_smoothing, spline_type=spline_type)
             g.flux[:] = [row[f] for f in filternames]
             g.ra, g.dec = float(row["ra"]), float(row["dec"])
             g.q, g.pa = float(row["q"]), float(row["pa"])
