@@ -3,10 +3,11 @@
 forcepho's ``tests/data/test_image`` fixture -- a GalSim rendering of one
 n = 1, r_h = 0.15 arcsec Sersic source of flux 1 on a 100 x 100 TAN image,
 HDU 0 = model + noise, HDU 1 = the uncertainty, HDU 2 = the noise, last HDU
-= the one-row catalog (FILTERS in its header).  Its PSF file is HDF5 (not
-readable here); the PSF it holds is one circular Gaussian of sigma 2.5
-pixels for each of the 9 Sersic radius bins, which is what
-:func:`psf_store` writes."""
+= the one-row catalog (FILTERS in its header).  Its PSF file
+(``sersic1.0_rhalf0.150_snr050_noise1_psf.h5``, group CLEAR, dataset
+``parameters`` [1, 9]) holds one circular Gaussian (amp 1, Cxx = Cyy = 6.25,
+Cxy = 0, no offset) for each of the 9 Sersic radius bins, as h5dump shows;
+:func:`psf_store` writes those rows."""
 import os
 
 import numpy as np
