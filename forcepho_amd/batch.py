@@ -55,6 +55,18 @@ class DeviceSplines:
             pass
 
 
+def _on_stream(theta, n, stream=None):
+    """``theta`` as a contiguous float64 device vector of length ``n``, kept
+    alive for ``stream``: a contiguous copy made here is freed when the
+    caller drops it, and torch's allocator may hand its memory to work on the
+    current stream while the library still reads it on ``stream``."""
+    assert theta.dtype == torch.float64 and theta.is_cuda and theta.numel() == n
+    theta = theta.contiguous()
+    if stream is not None:
+        theta.record_stream(stream)
+    return theta
+
+
 def _stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(int(s.cuda_stream))
@@ -174,8 +186,7 @@ class PatchBatch:
         return self.chi2, self.dchi2
 
     def theta_to_proposals(self, theta, splines, stream=None):
-        assert theta.dtype == torch.float64 and theta.is_cuda and theta.numel() == self.n_theta
-        theta = theta.contiguous()
+        theta = _on_stream(theta, self.n_theta, stream)
         _lib.check(_lib.lib().fpho_batch_theta_to_proposals_async(
             self.handle, splines.handle, _lib.dptr(theta), _lib.dptr(self.proposals),
             _stream_ptr(stream)))
@@ -186,8 +197,7 @@ class PatchBatch:
         (float64 on the device): the image Gaussians are built from FP64
         source parameters (the FP32 records are written to ``proposals``
         too)."""
-        assert theta.dtype == torch.float64 and theta.is_cuda and theta.numel() == self.n_theta
-        theta = theta.contiguous()
+        theta = _on_stream(theta, self.n_theta, stream)
         _lib.check(_lib.lib().fpho_batch_evaluate_theta_async(
             self.handle, splines.handle, _lib.dptr(theta), _lib.dptr(self.proposals),
             _lib.dptr(self.chi2), _lib.dptr(self.dchi2), _stream_ptr(stream)))
