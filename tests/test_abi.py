@@ -176,3 +176,23 @@ def test_measurement_library_is_separate():
     for name in ("fpho_issue_calibration", "fpho_batch_count_pairs"):
         assert not hasattr(lib, name)
     assert "calibrat" not in open(HEADER).read()
+
+
+def test_c_host_compiles_links_and_validates_its_input(tmp_path):
+    """The plain C host of the library (tests/c_host/evaluate_patch.c, used
+    on the GPU by test_gpu_c_host.py) compiles against include/fpho.h with
+    warnings as errors, resolves every library symbol it calls at load time
+    (LD_BIND_NOW), and rejects a bad command line or input directory before
+    it touches the device."""
+    lib = os.path.dirname(_lib.LIBPATH)
+    exe = tmp_path / "evaluate_patch"
+    subprocess.run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-I",
+                    os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c_host", "evaluate_patch.c"),
+                    "-L", lib, "-lfpho_hip", "-Wl,-rpath," + lib, "-o", str(exe)], check=True)
+    env = dict(os.environ, LD_BIND_NOW="1")
+    usage = subprocess.run([str(exe)], capture_output=True, text=True, env=env)
+    assert usage.returncode == 2 and "usage" in usage.stderr
+    missing = subprocess.run([str(exe), str(tmp_path / "absent")], capture_output=True,
+                             text=True, env=env)
+    assert missing.returncode == 2 and "meta.txt" in missing.stderr
