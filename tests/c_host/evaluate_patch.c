@@ -111,5 +111,11 @@ int main(int argc, char **argv) {
     write_file("dchi2.f32", dchi2, sizeof(float) * n_bands * n_sources * 7 * n_props);
     printf("evaluated %lld proposals of a %lld-pixel patch, %lld bands, %lld sources\n",
            n_props, npix, n_bands, n_sources);
+    const void *owned[] = {d.data, d.ierr, d.xpix, d.ypix, d.band_start, d.band_N,
+                           d.exposure_start, d.exposure_N, d.rad2, d.D, d.crpix, d.crval,
+                           d.CW, d.G, d.n_psf_per_source, d.psfgauss, d.psfgauss_start, props};
+    for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); ++i) free((void *)owned[i]);
+    free(chi2);
+    free(dchi2);
     return 0;
 }
