@@ -7,7 +7,8 @@ device copy reads FETCH_SIZE = 0.5 GiB and WRITE_SIZE = 1 GiB.
 
 Usage (each ``<dir>`` is a rocprofv3 ``-d`` output directory):
 
-    python profiling.py stats <dir>
+    python profiling.py stats <dir> [--limit 12]
+    python profiling.py api <dir> [--match Graph]
     python profiling.py counters <dir> [--min-ms 10]
     python profiling.py traffic --fetch <dir> --write <dir> \\
         --cal-fetch <dir> --cal-write <dir> --launch K,W,P,G --command "..."
@@ -35,7 +36,8 @@ KERNEL_SOURCES = [KERNEL_SOURCE, os.path.join(CSRC, "kernels.h"),
                   os.path.join(CSRC, "fpho_internal.h")]
 COPY_BYTES = 1 << 30   # bench.py --copy-calibration copies 1 GiB per dispatch
 
-__all__ = ["kernel_source_digest", "kernel_stats", "counter_medians", "read_summary"]
+__all__ = ["kernel_source_digest", "kernel_stats", "api_stats", "counter_medians",
+           "read_summary"]
 
 
 def _code_only(text):
@@ -72,6 +74,22 @@ def kernel_stats(path, limit=12):
                                  "percentage from top_kernels limit ?", (limit,)))
         con.close()
     return rows
+
+
+def api_stats(path, match=None):
+    """Rows (name, calls, total_us, avg_us) of the traced host API calls (the
+    ``regions`` view of a ``--hip-runtime-trace`` run), largest total first;
+    with ``match``, only the names containing it."""
+    acc = collections.defaultdict(lambda: [0, 0.0])
+    for db in _dbs(path):
+        con = sqlite3.connect(db)
+        for name, dur in con.execute("select name, duration from regions"):
+            if match is None or match in name:
+                acc[name][0] += 1
+                acc[name][1] += (dur or 0) / 1e3
+        con.close()
+    rows = [(n, c, t, t / c) for n, (c, t) in acc.items()]
+    return sorted(rows, key=lambda r: -r[2])
 
 
 def counter_medians(path, min_ms=0.0):
@@ -141,6 +159,10 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("path")
+    s.add_argument("--limit", type=int, default=12)
+    ap_ = sub.add_parser("api")
+    ap_.add_argument("path")
+    ap_.add_argument("--match", default=None)
     c = sub.add_parser("counters")
     c.add_argument("path")
     c.add_argument("--min-ms", type=float, default=0.0)
@@ -152,8 +174,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(f"{'avg_us':>12s} {'calls':>6s} {'total_us':>14s} {'pct':>6s}  kernel")
-        for name, calls, tot, avg, pct in kernel_stats(a.path):
+        for name, calls, tot, avg, pct in kernel_stats(a.path, a.limit):
             print(f"{avg:12.3f} {calls:6d} {tot:14.1f} {pct:6.2f}  {name[:110]}")
+    elif a.cmd == "api":
+        print(f"{'avg_us':>12s} {'calls':>6s} {'total_us':>14s}  call")
+        for name, calls, tot, avg in api_stats(a.path, a.match):
+            print(f"{avg:12.3f} {calls:6d} {tot:14.1f}  {name}")
     elif a.cmd == "counters":
         for (k, cn), (n, med, dur) in sorted(counter_medians(a.path, a.min_ms).items()):
             print(f"{k[:70]:70s} {cn:28s} n={n:5d} median={med:.6e} dur_ms={dur / 1e6:.3f}")

@@ -21,6 +21,10 @@ def prof_dir(tmp_path):
     con.execute("create table top_kernels (name text, total_calls int, total_duration real, "
                 "average real, percentage real)")
     con.execute("insert into top_kernels values ('k', 2, 10.0, 5.0, 100.0)")
+    con.execute("create table regions (name text, duration int)")
+    con.executemany("insert into regions values (?, ?)",
+                    [("hipGraphLaunch", 2000), ("hipGraphLaunch", 4000),
+                     ("hipLaunchKernel", 500), ("hipStreamBeginCapture", 1000)])
     con.commit()
     con.close()
     return str(d)
@@ -53,3 +57,10 @@ def test_kernel_digest_ignores_comments_and_whitespace(tmp_path):
     d = profiling.kernel_source_digest
     assert d([str(a)]) == d([str(b)])
     assert d([str(a)]) != d([str(c)])
+
+
+def test_api_stats_counts_calls_per_name(prof_dir):
+    rows = profiling.api_stats(prof_dir)
+    assert rows[0] == ("hipGraphLaunch", 2, 6.0, 3.0)
+    assert {r[0] for r in rows} == {"hipGraphLaunch", "hipLaunchKernel", "hipStreamBeginCapture"}
+    assert [r[0] for r in profiling.api_stats(prof_dir, "Graph")] == ["hipGraphLaunch"]
