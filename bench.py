@@ -56,7 +56,7 @@ def parse():
                     help="target length of the CPU baseline sample")
     ap.add_argument("--launch", type=str, default=None,
                     help="K,W,persistent,groups_per_cu (default: autotuner cache)")
-    ap.add_argument("--precision", default="mixed", choices=["fp32", "fp64_reduce", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp32", "fp64_reduce", "mixed", "mixed_centre"])
     ap.add_argument("--tune", type=str, default=None, metavar="CACHE_JSON",
                     help="run the launch autotuner on this workload, write the "
                          "cache to CACHE_JSON and exit")

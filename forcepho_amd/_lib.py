@@ -40,7 +40,7 @@ EXPORTED = [
 ]
 
 ARRAY_IDS = {"data": 0, "ierr": 1, "xpix": 2, "ypix": 3, "residual": 4}
-PRECISION = {"fp32": 0, "fp64_reduce": 1, "mixed": 2}
+PRECISION = {"fp32": 0, "fp64_reduce": 1, "mixed": 2, "mixed_centre": 3}
 # FPHO_STATUS_* bits of the device status word
 STATUS_BITS = {0x1: "non-finite proposal value", 0x2: "axis ratio q <= 0",
                0x4: "image Gaussian covariance not positive definite",

@@ -58,7 +58,7 @@ struct alignas(16) ImgGauss {
     float cA_rh, k_axx, k_axy, k_e;          // -amp/(48s^2), -amp/(24s^2), -amp/24
     float k_u, k_x1, k_x2, dFxx_dQ;          // amp/(12s), amp/(24s^3), -amp/(12s^2)
     float dFyy_dQ, dFxy_dQ, dFxx_dPA, dFyy_dPA;
-    float dFxy_dPA, pad1, pad2, pad3;
+    float dFxy_dPA, xcen_lo, ycen_lo, pad1;  // centre = xcen + xcen_lo (FP64 to ~1e-14)
 };
 static_assert(sizeof(ImgGauss) == 128, "ImgGauss must be 128 bytes");
 
@@ -163,6 +163,7 @@ enum Precision : int32_t {
     kPrecFP64Reduce = 1,  // FP32 residual (as the reference kernel), FP64 sums
     kPrecMixed = 2,       // FP64 residual accumulation, bright terms in FP64,
                           // FP64 sums (default)
+    kPrecMixedCentre = 3, // kPrecMixed with pixel offsets from the hi/lo centre
 };
 // Terms brighter than kBrightSigma * sigma_pixel are evaluated in FP64.
 constexpr float kBrightSigma = 0.1f;

@@ -132,7 +132,9 @@ __device__ __forceinline__ uint32_t make_image_gaussian(const ExpDesc &e, const 
     g.dFxx_dPA = (float)dFp.a;
     g.dFyy_dPA = (float)dFp.d;
     g.dFxy_dPA = (float)dFp.c;
-    g.pad1 = g.pad2 = g.pad3 = 0.0f;
+    g.xcen_lo = (float)(xc - (double)g.xcen);
+    g.ycen_lo = (float)(yc - (double)g.ycen);
+    g.pad1 = 0.0f;
 
     g64.xcen = xc;
     g64.ycen = yc;
@@ -259,7 +261,7 @@ __device__ __forceinline__ double pair_value64(const ImgGauss64 &g, float xp, fl
 // for the first 64 act_words Gaussians); pass 2 visits only those Gaussians
 // and evaluates only those slots.  Gaussians past either capacity are
 // visited with every slot: a pixel beyond the cutoff adds exact zeros.
-template <int K, bool MIXED>
+template <int K, bool MIXED, bool CENTRE>
 __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__ tiles,
                                           const ExpDesc *__restrict__ exps,
                                           const ImgGauss *__restrict__ gauss,
@@ -349,6 +351,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
             const uint32_t smi = __builtin_amdgcn_readlane(sm, bit);
             uint32_t m = 0;   // slots with a pixel inside the cutoff (wave-uniform)
             const float xc = g[i].xcen, yc = g[i].ycen;
+            const float xcl = CENTRE ? g[i].xcen_lo : 0.0f, ycl = CENTRE ? g[i].ycen_lo : 0.0f;
             const float sfxx = g[i].sfxx, sfyy = g[i].sfyy, sfxy = g[i].sfxy;
             const float hcs = g[i].hcs, ampA = g[i].ampA;
             // a term is bright when |c| > tb: a Gaussian whose peak stays
@@ -357,8 +360,12 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if (!(smi & (1u << k))) continue;
-                const float dx = x[k] - xc;
-                const float dy = y[k] - yc;
+                float dx = x[k] - xc;
+                float dy = y[k] - yc;
+                if (CENTRE) {   // the pixel offset from the FP64 centre
+                    dx -= xcl;
+                    dy -= ycl;
+                }
                 const float vx = sfxx * dx + sfxy * dy;
                 const float vy = sfyy * dy + sfxy * dx;
                 const float ea = dx * vx + dy * vy;      // s E
@@ -455,6 +462,7 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
                 ? (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)amask[i]) : 0xFFu;
             touched = true;
             const float xc = G.xcen, yc = G.ycen;
+            const float xcl = CENTRE ? G.xcen_lo : 0.0f, ycl = CENTRE ? G.ycen_lo : 0.0f;
             const float sfxx = G.sfxx, sfyy = G.sfyy, sfxy = G.sfxy, hcs = G.hcs;
             // the 12 per-Gaussian pixel sums (over this lane's K pixels)
             float PA = 0.f, X1 = 0.f, X2 = 0.f, Y1 = 0.f, Y2 = 0.f, Pxx = 0.f;
@@ -462,8 +470,12 @@ __device__ __forceinline__ void eval_tile(int tile, const TileDesc *__restrict__
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if (!(m & (1u << k))) continue;
-                const float dx = x[k] - xc;
-                const float dy = y[k] - yc;
+                float dx = x[k] - xc;
+                float dy = y[k] - yc;
+                if (CENTRE) {
+                    dx -= xcl;
+                    dy -= ycl;
+                }
                 const float vx = sfxx * dx + sfxy * dy;
                 const float vy = sfyy * dy + sfxy * dx;
                 const float u = dx * vx;
@@ -563,7 +575,7 @@ __device__ __forceinline__ int xcd_swizzle(int b, int n) {
 #define FPHO_EVAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
 
 // Non-persistent: wave w of workgroup b evaluates queue[b*W + w].
-template <int K, bool MIXED>
+template <int K, bool MIXED, bool CENTRE>
 __global__ __launch_bounds__(256) FPHO_EVAL_OCCUPANCY void eval_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t waves_per_group, const ExpDesc *__restrict__ exps,
@@ -578,15 +590,15 @@ __global__ __launch_bounds__(256) FPHO_EVAL_OCCUPANCY void eval_kernel(
                      waves_per_group + w;
     if (slot >= n_tiles) return;
     const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-    eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gboxes, slot_boxes, gpart, cpart, cull,
-                        lds_mask + w * mask_cap, mask_cap,
+    eval_tile<K, MIXED, CENTRE>(tile, tiles, exps, gauss, gauss64, gboxes, slot_boxes, gpart, cpart,
+                        cull, lds_mask + w * mask_cap, mask_cap,
                         reinterpret_cast<uint64_t *>(lds_mask + waves_per_group * mask_cap) +
                             w * act_words, act_words);
 }
 
 // Persistent: every wave pulls tiles from `counter` until the queue is empty.
 // Each wave reaches the exit once the counter passes n_tiles.
-template <int K, bool MIXED>
+template <int K, bool MIXED, bool CENTRE>
 __global__ __launch_bounds__(256) FPHO_EVAL_OCCUPANCY void eval_persistent_kernel(
     const TileDesc *__restrict__ tiles, const int32_t *__restrict__ queue,
     int32_t n_tiles, int32_t *__restrict__ counter, const ExpDesc *__restrict__ exps,
@@ -605,8 +617,8 @@ __global__ __launch_bounds__(256) FPHO_EVAL_OCCUPANCY void eval_persistent_kerne
         slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, kWave));
         if (slot >= n_tiles) break;
         const int tile = __builtin_amdgcn_readfirstlane(queue[slot]);
-        eval_tile<K, MIXED>(tile, tiles, exps, gauss, gauss64, gboxes, slot_boxes, gpart, cpart, cull,
-                            lds_mask + w * mask_cap, mask_cap,
+        eval_tile<K, MIXED, CENTRE>(tile, tiles, exps, gauss, gauss64, gboxes, slot_boxes, gpart,
+                            cpart, cull, lds_mask + w * mask_cap, mask_cap,
                             reinterpret_cast<uint64_t *>(lds_mask + W * mask_cap) +
                                 w * act_words, act_words);
     }
@@ -887,20 +899,20 @@ hipError_t launch_response(const double *dchi2, int32_t n_bands, int32_t n_src,
     return hipGetLastError();
 }
 
-template <int K, bool MIXED>
+template <int K, bool MIXED, bool CENTRE>
 static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
     const int W = a.waves_per_group;
     // per wave: the slot mask bytes, then the activity words
     const size_t lds = (size_t)W * ((size_t)a.mask_cap + 8 * (size_t)a.act_words);
     if (a.persistent) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
-        hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED>), dim3((unsigned)a.grid),
+        hipLaunchKernelGGL((eval_persistent_kernel<K, MIXED, CENTRE>), dim3((unsigned)a.grid),
                            dim3(64 * W), lds, stream, a.tiles, a.queue, (int32_t)a.n_tiles,
                            a.counter, a.exps, a.gauss, a.gauss64, a.gboxes, a.slot_boxes, a.gpart, a.cpart,
                            (int32_t)a.cull, (int32_t)a.mask_cap, (int32_t)a.act_words);
     } else {
         const int64_t groups = (a.n_tiles + W - 1) / W;
-        hipLaunchKernelGGL((eval_kernel<K, MIXED>), dim3((unsigned)groups), dim3(64 * W), lds,
+        hipLaunchKernelGGL((eval_kernel<K, MIXED, CENTRE>), dim3((unsigned)groups), dim3(64 * W), lds,
                            stream, a.tiles, a.queue, (int32_t)a.n_tiles, (int32_t)W, a.exps,
                            a.gauss, a.gauss64, a.gboxes, a.slot_boxes, a.gpart, a.cpart, (int32_t)a.cull,
                            (int32_t)a.mask_cap, (int32_t)a.act_words);
@@ -908,21 +920,24 @@ static hipError_t launch_eval_k(const EvalArgs &a, hipStream_t stream) {
     return hipGetLastError();
 }
 
-template <bool MIXED>
+template <bool MIXED, bool CENTRE>
 static hipError_t launch_eval_p(const EvalArgs &a, hipStream_t stream) {
     switch (a.pix_per_lane) {
-        case 1: return launch_eval_k<1, MIXED>(a, stream);
-        case 2: return launch_eval_k<2, MIXED>(a, stream);
-        case 4: return launch_eval_k<4, MIXED>(a, stream);
-        case 8: return launch_eval_k<8, MIXED>(a, stream);
+        case 1: return launch_eval_k<1, MIXED, CENTRE>(a, stream);
+        case 2: return launch_eval_k<2, MIXED, CENTRE>(a, stream);
+        case 4: return launch_eval_k<4, MIXED, CENTRE>(a, stream);
+        case 8: return launch_eval_k<8, MIXED, CENTRE>(a, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_eval(const EvalArgs &a, hipStream_t stream) {
     if (a.n_tiles <= 0) return hipSuccess;
-    return a.precision == kPrecMixed ? launch_eval_p<true>(a, stream)
-                                     : launch_eval_p<false>(a, stream);
+    switch (a.precision) {
+        case kPrecMixed: return launch_eval_p<true, false>(a, stream);
+        case kPrecMixedCentre: return launch_eval_p<true, true>(a, stream);
+        default: return launch_eval_p<false, false>(a, stream);
+    }
 }
 
 hipError_t launch_reduce(const OutDesc *outs, int64_t n_outs, const float *gpart,

@@ -913,7 +913,7 @@ int fpho_patch_set_launch(fpho_patch *p, const fpho_launch *l) {
 
 int fpho_patch_set_precision(fpho_patch *p, int mode) {
     if (!p) return fail(FPHO_E_ARG, "null patch");
-    if (mode < kPrecFP32 || mode > kPrecMixed) return fail(FPHO_E_ARG, "precision mode %d", mode);
+    if (mode < kPrecFP32 || mode > kPrecMixedCentre) return fail(FPHO_E_ARG, "precision mode %d", mode);
     p->precision = mode;
     p->plan.precision = mode;
     return FPHO_OK;
@@ -1102,7 +1102,7 @@ int fpho_batch_set_launch(fpho_batch *b, const fpho_launch *l) {
 
 int fpho_batch_set_precision(fpho_batch *b, int mode) {
     if (!b) return fail(FPHO_E_ARG, "null batch");
-    if (mode < kPrecFP32 || mode > kPrecMixed) return fail(FPHO_E_ARG, "precision mode %d", mode);
+    if (mode < kPrecFP32 || mode > kPrecMixedCentre) return fail(FPHO_E_ARG, "precision mode %d", mode);
     b->precision = mode;
     b->plan.precision = mode;
     return FPHO_OK;

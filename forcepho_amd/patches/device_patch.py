@@ -246,7 +246,8 @@ class HIPPatchMixin(DevicePatchMixin):
         _lib.check(_lib.lib().fpho_patch_set_launch(self.device_patch, ctypes.byref(l)))
 
     def set_precision(self, mode):
-        """"fp32", "fp64_reduce" or "mixed" (default)."""
+        """"fp32", "fp64_reduce", "mixed" (default) or "mixed_centre" (the
+        mixed mode with every pixel offset taken from the FP64 centre)."""
         _lib.check(_lib.lib().fpho_patch_set_precision(self.device_patch,
                                                        _lib.PRECISION[mode]))
 

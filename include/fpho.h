@@ -62,10 +62,17 @@ extern "C" {
  *                         (common_kernel.cc:57-83), FP64 cross-tile sums
  *   FPHO_PREC_MIXED       FP64 residual accumulation; terms brighter than
  *                         0.1 sigma of their pixel re-evaluated in FP64;
- *                         FP64 cross-tile sums (default) */
+ *                         FP64 cross-tile sums (default)
+ *   FPHO_PREC_MIXED_CENTRE FPHO_PREC_MIXED with every pixel offset taken from
+ *                         the FP64 centre (FP32 high part, then the low
+ *                         part), so the FP32 terms and the gradient pass
+ *                         see the centre the FP64 bright terms see: position
+ *                         derivatives that nearly cancel over the pixels
+ *                         keep their accuracy; about 8% slower */
 #define FPHO_PREC_FP32 0
 #define FPHO_PREC_FP64_REDUCE 1
 #define FPHO_PREC_MIXED 2
+#define FPHO_PREC_MIXED_CENTRE 3
 
 /* == class Source, forcepho/src/proposal.cu:13-27 (264 bytes). */
 typedef struct fpho_source {

@@ -66,7 +66,7 @@ def test_launch_shapes_within_tolerance(patches, K, W):
     batch.free()
 
 
-@pytest.mark.parametrize("mode", ["fp32", "fp64_reduce", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "fp64_reduce", "mixed", "mixed_centre"])
 def test_precision_modes(patches, mode):
     props = [proposal_for(sc, th) for _, sc, th in patches]
     batch = PatchBatch([p for p, _, _ in patches], precision=mode)
@@ -75,7 +75,7 @@ def test_precision_modes(patches, mode):
     for (p, sc, th), prop, c, d in zip(patches, props, chi2, dchi2):
         c_ref, d_ref = oracle.evaluate(p, prop)
         worst = max(worst, chi2_relerr(c, c_ref).max())
-        if mode == "mixed":
+        if mode in ("mixed", "mixed_centre"):
             assert not grad_mismatch(d, d_ref).any()
         else:
             scale = np.abs(d_ref).max()
@@ -90,7 +90,7 @@ def test_box_culling_changes_no_bit(patches):
     results identical to evaluating every pair."""
     props = [proposal_for(sc, th) for _, sc, th in patches]
     batch = PatchBatch([p for p, _, _ in patches])
-    for mode in ("mixed", "fp64_reduce"):
+    for mode in ("mixed", "fp64_reduce", "mixed_centre"):
         batch.set_precision(mode)
         batch.set_launch(cull=1)
         c1, d1 = _batch_eval(batch, props)

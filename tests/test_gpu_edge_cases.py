@@ -122,3 +122,51 @@ def test_amplitude_edge_branches_on_device(hip):
         splines.free()
         batch.free()
         patch.free()
+
+
+def _edge_theta(theta, nb):
+    from forcepho_amd.spline import get_splines
+    t = theta.copy().reshape(4, nb + 6)
+    t[0, nb + 4] = get_splines().sersic_range[0] - 0.1   # below the table
+    t[1, nb + 5] = -0.01                                   # rh < 0
+    return t.reshape(-1)
+
+
+def test_exact_centre_mode_on_a_near_cancelling_position_derivative(hip):
+    """The amplitude edge branches against data rendered from the truth: the
+    point-like source 1 (rh < 0) sits away from its data, and its position
+    derivative is a sum that nearly cancels over the pixels.  In the mixed
+    mode the FP64 bright terms and the FP32 terms see centres up to half an
+    FP32 ulp apart, which moves such a component by up to ~3e-4 relative;
+    the mixed_centre mode takes every pixel offset from the FP64 centre, and
+    every component meets the stated tolerance."""
+    patch, scene, theta = _patch(63)
+    nb = len(patch.band_N)
+    t = _edge_theta(theta, nb)
+    patch.send_to_device()
+    splines = DeviceSplines()
+    ll_ref, g_ref = oracle.lnlike_theta(patch, t)
+    tol = lambda g: (np.abs(g - g_ref) <= 1e-4 * np.abs(g_ref)) | \
+        (np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref).max())
+    try:
+        tt = torch.as_tensor(t, dtype=torch.float64, device="cuda")
+        out = {}
+        for mode in ("mixed", "mixed_centre"):
+            batch = PatchBatch([patch], precision=mode)
+            try:
+                ll, g = batch.lnlike_and_grad(tt, splines)
+                out[mode] = (ll.item(), g.cpu().numpy())
+            finally:
+                batch.free()
+        for mode, (ll, g) in out.items():
+            nz = g_ref != 0
+            rel = np.abs(g - g_ref)[nz] / np.abs(g_ref)[nz]
+            print(f"{mode}: lnlike rel err {abs(ll - ll_ref) / abs(ll_ref):.2e}, "
+                  f"{int((~tol(g)).sum())} components outside tolerance, "
+                  f"worst rel err of the nonzero ones {rel.max():.2e}", flush=True)
+        ll, g = out["mixed_centre"]
+        assert abs(ll - ll_ref) <= CHI2_RTOL * abs(ll_ref)
+        assert tol(g).all()
+    finally:
+        splines.free()
+        patch.free()
