@@ -133,11 +133,13 @@ def host_cpu():
             "affinity": len(os.sched_getaffinity(0))}
 
 
-def measured_traffic(launch):
+def measured_traffic(launch, precision="mixed"):
     """HBM bytes per pixel-pass launch from the committed rocprofv3
-    byte-counter summary, if it was taken on this kernel source and launch
-    shape; else (None, reason)."""
+    byte-counter summary, if it was taken on this kernel source, launch
+    shape and precision mode (the default, mixed); else (None, reason)."""
     import profiling
+    if precision != "mixed":
+        return None, f"byte-counter summary taken in the mixed mode, not {precision}"
     if not os.path.exists(TRAFFIC_SUMMARY):
         return None, "no byte-counter summary"
     kv = profiling.read_summary(TRAFFIC_SUMMARY)
@@ -555,7 +557,7 @@ def main():
                        "pairs_per_s": c["pairs_per_s"]}
             log(f"cpu reference, {workers} processes: {c['pairs_per_s']:.3e} pairs/s")
 
-    traffic, traffic_src = measured_traffic(launch)
+    traffic, traffic_src = measured_traffic(launch, args.precision)
     # exp-issue ceiling: one exp per in-cutoff pair (the algorithmic count)
     # at the device's measured exp throughput, as a time per sweep
     exp_rate = exp_issue_rate()
