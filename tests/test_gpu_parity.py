@@ -77,10 +77,12 @@ def test_config2_truth(c2):
     _check(patch, proposal_for(scene, theta), "c2 truth")
 
 
-def test_config2_100_proposals_against_oracle_fixtures(c2):
+@pytest.mark.parametrize("mode", ["mixed", "mixed_centre"])
+def test_config2_100_proposals_against_oracle_fixtures(c2, mode):
     """The 100 seeded proposals of configuration 2 against FP64 oracle chi2
     and gradients committed in tests/golden/config2_proposals.npz (written
-    by tests/golden/make_scale_fixtures.py on the same seeded patch)."""
+    by tests/golden/make_scale_fixtures.py on the same seeded patch), in the
+    default mode and in the exact-centre mode."""
     patch, scene, theta = c2
     fx = load_fixture(FIXTURE_C2)
     assert Fixture.sha256(patch.data) == fx["data_sha256"], \
@@ -89,17 +91,21 @@ def test_config2_100_proposals_against_oracle_fixtures(c2):
     assert Fixture.sha256(np.concatenate(props)) == fx["proposals_sha256"]
     proposer = patch.get_proposer()
     worst_c, worst_g = 0.0, 0.0
-    for k, prop in enumerate(props):
-        chi2, dchi2 = proposer.evaluate_proposal(prop, patch=patch)[:2]
-        c_ref, d_ref = fx["chi2"][k], fx["dchi2"][k]
-        rel = chi2_relerr(chi2, c_ref)
-        bad = grad_mismatch(dchi2, d_ref)
-        scale = np.abs(d_ref).reshape(len(d_ref), -1).max(axis=1)[:, None, None]
-        worst_c = max(worst_c, rel.max())
-        worst_g = max(worst_g, float(np.max(np.abs(dchi2 - d_ref) / scale)))
-        assert rel.max() <= CHI2_RTOL, (k, rel)
-        assert not bad.any(), (k, np.argwhere(bad)[:5])
-    print(f"c2, 100 proposals: chi2 rel err <= {worst_c:.2e}; gradient err <= "
+    patch.set_precision(mode)
+    try:
+        for k, prop in enumerate(props):
+            chi2, dchi2 = proposer.evaluate_proposal(prop, patch=patch)[:2]
+            c_ref, d_ref = fx["chi2"][k], fx["dchi2"][k]
+            rel = chi2_relerr(chi2, c_ref)
+            bad = grad_mismatch(dchi2, d_ref)
+            scale = np.abs(d_ref).reshape(len(d_ref), -1).max(axis=1)[:, None, None]
+            worst_c = max(worst_c, rel.max())
+            worst_g = max(worst_g, float(np.max(np.abs(dchi2 - d_ref) / scale)))
+            assert rel.max() <= CHI2_RTOL, (k, rel)
+            assert not bad.any(), (k, np.argwhere(bad)[:5])
+    finally:
+        patch.set_precision("mixed")
+    print(f"c2 ({mode}), 100 proposals: chi2 rel err <= {worst_c:.2e}; gradient err <= "
           f"{worst_g:.2e} of the band max; all 112000 components within tolerance",
           flush=True)
 
