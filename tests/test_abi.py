@@ -196,3 +196,15 @@ def test_c_host_compiles_links_and_validates_its_input(tmp_path):
     missing = subprocess.run([str(exe), str(tmp_path / "absent")], capture_output=True,
                              text=True, env=env)
     assert missing.returncode == 2 and "meta.txt" in missing.stderr
+
+
+def test_precision_modes_match_the_header():
+    """forcepho_amd._lib.PRECISION names every FPHO_PREC_* mode of
+    include/fpho.h with its value; setting one on a null patch is an
+    argument error before any device work."""
+    text = open(HEADER).read()
+    modes = {m.group(1).lower(): int(m.group(2))
+             for m in re.finditer(r"#define FPHO_PREC_([A-Z0-9_]+) (\d+)", text)}
+    assert modes == _lib.PRECISION
+    z = ctypes.c_void_p(0)
+    assert _lib.lib().fpho_patch_set_precision(z, 0) == 1   # null patch: FPHO_E_ARG
