@@ -2,8 +2,10 @@
  * reads a packed patch and proposals from raw little-endian files in the
  * directory argv[1] (written by tests/test_gpu_c_host.py), creates the
  * device patch with fpho_patch_create, evaluates every proposal with the
- * synchronous drop-in fpho_evaluate, and writes chi2 (double) and dchi2
- * (float) back to that directory.  This is the call sequence a C, Go or
+ * synchronous drop-in fpho_evaluate and, when the sources fit the
+ * reference's MAXSOURCES, with its compat form fpho_evaluate_response, and
+ * writes chi2 (double), dchi2 (float) and the Response blocks back to that
+ * directory.  This is the call sequence a C, Go or
  * Julia binding of forcepho's GPUProposer.evaluate_proposal makes
  * (forcepho/proposal.py:96-152). */
 #include <stdint.h>
@@ -106,9 +108,22 @@ int main(int argc, char **argv) {
     for (long long k = 0; k < n_props; ++k)
         CHECK(fpho_evaluate(p, props + k * n_sources, chi2 + k * n_bands,
                             dchi2 + k * n_bands * n_sources * 7));
+    /* the same proposals through the compat output: Response[n_bands] per
+     * proposal (forcepho/src/proposal.cu:33-39), for patches whose sources
+     * fit its MAXSOURCES rows */
+    long long n_resp = n_sources <= FPHO_MAXSOURCES ? n_props : 0;
+    double *chi2_r = malloc(sizeof(double) * n_bands * (n_resp > 0 ? n_resp : 1));
+    fpho_response *resp = malloc(sizeof(fpho_response) * n_bands * (n_resp > 0 ? n_resp : 1));
+    for (long long k = 0; k < n_resp; ++k)
+        CHECK(fpho_evaluate_response(p, props + k * n_sources, chi2_r + k * n_bands,
+                                     resp + k * n_bands));
     CHECK(fpho_patch_free(p));
     write_file("chi2.f64", chi2, sizeof(double) * n_bands * n_props);
     write_file("dchi2.f32", dchi2, sizeof(float) * n_bands * n_sources * 7 * n_props);
+    write_file("chi2_response.f64", chi2_r, sizeof(double) * n_bands * n_resp);
+    write_file("response.bin", resp, sizeof(fpho_response) * n_bands * n_resp);
+    free(chi2_r);
+    free(resp);
     printf("evaluated %lld proposals of a %lld-pixel patch, %lld bands, %lld sources\n",
            n_props, npix, n_bands, n_sources);
     const void *owned[] = {d.data, d.ierr, d.xpix, d.ypix, d.band_start, d.band_N,

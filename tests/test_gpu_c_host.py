@@ -1,7 +1,8 @@
 """libfpho_hip.so driven by a plain C program, with no Python or torch in its
 process (tests/c_host/evaluate_patch.c: fpho_init, fpho_patch_create,
-fpho_evaluate per proposal, fpho_patch_free -- the calls a C binding of
-forcepho's GPUProposer.evaluate_proposal makes, forcepho/proposal.py:96-152).
+fpho_evaluate and the compat fpho_evaluate_response per proposal,
+fpho_patch_free -- the calls a C binding of forcepho's
+GPUProposer.evaluate_proposal makes, forcepho/proposal.py:96-152).
 Its results must equal the Python drop-in's bit for bit (one library, a
 fixed tile partition and reduction order) and match the FP64 oracle.  The
 program is built by __graft_entry__.build() with the library."""
@@ -13,7 +14,7 @@ import pytest
 
 import oracle
 from forcepho_amd import synth
-from forcepho_amd.limits import psf_struct_dtype, source_struct_dtype
+from forcepho_amd.limits import MAXSOURCES, psf_struct_dtype, source_struct_dtype
 from helpers import CHI2_RTOL, chi2_relerr, grad_mismatch, make_config_patch, proposal_for
 
 pytestmark = pytest.mark.gpu
@@ -64,6 +65,15 @@ def test_c_host_equals_python_dropin(hip, tmp_path, config):
             chi2, dchi2 = proposer.evaluate_proposal(prop, patch=patch)[:2]
             assert np.array_equal(chi2_c[k], chi2), k
             assert np.array_equal(dchi2_c[k], dchi2), k
+        # the compat Response output: rows [0, n_sources) of each band equal
+        # the compact gradients bit for bit, the rows past them are zero
+        chi2_r = np.fromfile(tmp_path / "chi2_response.f64",
+                             dtype=np.float64).reshape(len(props), nb)
+        resp = np.fromfile(tmp_path / "response.bin", dtype=np.float32).reshape(
+            len(props), nb, MAXSOURCES, 7)
+        assert np.array_equal(chi2_r, chi2_c)
+        assert np.array_equal(resp[:, :, :ns], dchi2_c)
+        assert not resp[:, :, ns:].any()
         c_ref, d_ref = oracle.evaluate(patch, props[0])
         assert chi2_relerr(chi2_c[0], c_ref).max() <= CHI2_RTOL
         assert not grad_mismatch(dchi2_c[0], d_ref).any()
